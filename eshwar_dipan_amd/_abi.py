@@ -1,0 +1,94 @@
+"""ctypes mirror of include/gpstep.h and include/gpio.h.
+
+The structures here must match the C declarations field for field; the tests
+check the sizes against the library (gp_abi_sizes) so a drift fails loudly.
+"""
+import ctypes as ct
+
+GP_MAX_PHASES = 16
+GP_MAX_COMP = 4
+GP_NAME_LEN = 51
+
+GP_OK = 0
+GP_F64 = 0
+GP_F32 = 1
+
+GP_MU_GENERAL, GP_MU_BINARY, GP_MU_TERNARY, GP_MU_DILUTE = 0, 1, 2, 3
+
+_D = ct.c_double
+PM, CM = GP_MAX_PHASES, GP_MAX_COMP
+
+
+def _arr(t, *dims):
+    for d in reversed(dims):
+        t = t * d
+    return t
+
+
+class GpParams(ct.Structure):
+    _fields_ = [
+        ("mesh_x", ct.c_long), ("mesh_y", ct.c_long),
+        ("dx", _D), ("dy", _D), ("dt", _D),
+        ("nphases", ct.c_int), ("ncomp", ct.c_int),
+        ("obstacle", ct.c_int), ("anisotropy", ct.c_int),
+        ("mu_solver", ct.c_int), ("isothermal", ct.c_int),
+        ("epsilon", _D), ("tau", _D), ("T", _D), ("Teq", _D),
+        ("bc_type", _arr(ct.c_int, 2, 4)),
+        ("bc_proxy", _arr(ct.c_long, 2, 4, 3)),
+        ("gamma", _arr(_D, PM, PM)),
+        ("gamma_abc", _arr(_D, PM, PM, PM)),
+        ("tau_ab", _arr(_D, PM, PM)),
+        ("dab", _arr(_D, PM, PM)),
+        ("rot", _arr(_D, PM, PM, 3, 3)),
+        ("rot_inv", _arr(_D, PM, PM, 3, 3)),
+        ("diffusivity", _arr(_D, PM, CM, CM)),
+        ("A", _arr(_D, PM, CM, CM)),
+        ("ceq", _arr(_D, PM, PM, CM)),
+        ("cfill", _arr(_D, PM, PM, CM)),
+        ("slopes", _arr(_D, PM, PM, CM)),
+        ("cmu", _arr(_D, PM, CM, CM)),
+        ("Beq", _arr(_D, PM, CM)),
+        ("dBbdT", _arr(_D, PM, CM)),
+        ("B", _arr(_D, PM, CM)),
+        ("Cf", _arr(_D, PM)),
+    ]
+
+
+class GpStats(ct.Structure):
+    _fields_ = [
+        ("phi_max", _arr(_D, PM)), ("phi_min", _arr(_D, PM)),
+        ("mu_max", _arr(_D, CM)), ("mu_min", _arr(_D, CM)),
+        ("phi_sum", _arr(_D, PM)),
+        ("solute", _arr(_D, CM)),
+        ("nonfinite", _D),
+    ]
+
+
+class GpInput(ct.Structure):
+    _fields_ = [
+        ("p", GpParams),
+        ("dimension", ct.c_int),
+        ("mesh_z", ct.c_long),
+        ("dz", _D),
+        ("ntimesteps", ct.c_long), ("saveT", ct.c_long), ("nsmooth", ct.c_long),
+        ("starttime", ct.c_long), ("restart", ct.c_long), ("track_progress", ct.c_long),
+        ("ascii", ct.c_int), ("writecomposition", ct.c_int),
+        ("shift", ct.c_int), ("shiftj", ct.c_int),
+        ("noise_phasefield", ct.c_int),
+        ("amp_noise_phase", _D),
+        ("function_f", ct.c_int), ("function_w", ct.c_int),
+        ("function_anisotropy", ct.c_int), ("fold", ct.c_int),
+        ("tempgrady", ct.c_int),
+        ("tempgrad", _arr(_D, 5)),
+        ("Tfill", _D), ("R", _D), ("V", _D), ("temperature_scale", _D),
+        ("bc_value", _arr(_D, 3, 4)),
+        ("bc_type_T", _arr(ct.c_int, 4)),
+        ("have_gamma", ct.c_int), ("have_tau", ct.c_int),
+        ("have_dab", ct.c_int), ("have_gamma_abc", ct.c_int),
+        ("rot_given", _arr(ct.c_int, PM, PM)),
+        ("bc_proxy_set", _arr(ct.c_int, 2, 4)),
+        ("ncomponents_names", ct.c_int), ("nphases_names", ct.c_int),
+        ("phases", _arr(ct.c_char, PM, GP_NAME_LEN)),
+        ("components", _arr(ct.c_char, CM + 1, GP_NAME_LEN)),
+        ("unknown_keys", ct.c_int),
+    ]

@@ -1,0 +1,803 @@
+/*
+ * gp_host.c -- host side of the drop-in: Input.in / Filling.in parsing, constant
+ * derivation, initial fill and legacy-VTK I/O for the grand-potential 2-D step.
+ *
+ * Behaviour follows the reference serial solver (paths relative to
+ * Grand_potential_Finite_difference_2D_serial/):
+ *   parser ............ functions/reading_input_parameters.h:1-306,
+ *                       functions/read_boundary_conditions.h:4-42,
+ *                       functions/utility_functions.h:84-653
+ *   constants ......... functions/functionF_01.h:58-104,169-230,
+ *                       functions/initialize_variables.h:4-30, functions/matrix.h:16-218,
+ *                       functions/utility_functions.h:655-774 (rotations)
+ *   fill .............. functions/fill_domain.h:4-151, functions/filling.h:6-270
+ *   VTK ............... solverloop/file_writer.h:11-416
+ * Every constant that reaches the step is produced by the same sequence of
+ * binary64 operations as the reference, so that device and oracle steps start
+ * from bit-identical tables.
+ */
+#include "gpio.h"
+
+#include <ctype.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* The reference reads "%100s = %100[^;];" from 1000-byte lines
+   (reading_input_parameters.h:24-25): values longer than 100 characters are cut
+   there and the missing table entries are read from uninitialised memory.  This
+   parser uses the same grammar with room for 4000-character values, so every
+   input the reference reads correctly is read identically, and the long pair /
+   triple tables of many-phase runs are read in full. */
+#define GP_LINE_LEN 8192
+#define GP_VAL_LEN 4096
+#define GP_VAL_FMT "4000"
+
+#define P_MAX GP_MAX_PHASES
+#define C_MAX GP_MAX_COMP
+
+static void set_err(char *err, size_t n, const char *fmt, const char *a) {
+  if (err && n) snprintf(err, n, fmt, a ? a : "");
+}
+
+/* ------------------------------------------------------------------------ */
+/* Small dense linear algebra (matrix.h:16-218)                              */
+/* ------------------------------------------------------------------------ */
+
+/* Partial pivoting step.  The running maximum is compared by magnitude but,
+   as in matrix.h:161-191, replaced by the signed entry once a larger one is
+   found. */
+static void lu_pivot(double *m, double *fac, int k, int *tag, int n) {
+  double big = fabs(m[k * n + k]);
+  int t1 = k;
+  for (int i = k + 1; i < n; i++) {
+    if (fabs(m[i * n + k]) > big) {
+      t1 = i;
+      big = m[i * n + k];
+    }
+  }
+  int ts = tag[k];
+  tag[k] = tag[t1];
+  tag[t1] = ts;
+  for (int i = 0; i < n; i++) {
+    double s = m[k * n + i];
+    m[k * n + i] = m[t1 * n + i];
+    m[t1 * n + i] = s;
+  }
+  for (int i = 0; i < k; i++) {
+    double s = fac[k * n + i];
+    fac[k * n + i] = fac[t1 * n + i];
+    fac[t1 * n + i] = s;
+  }
+}
+
+void gp_matinv(double *a, double *inv, int n) {
+  double fac[16], inv1[16], prod[16], vec1[4], vec[4];
+  int tag[4];
+  memset(fac, 0, sizeof(fac));
+  for (int k = 0; k < n; k++) tag[k] = k;
+  /* elimination to upper triangular form, multipliers kept in fac */
+  for (int k = 0; k < n; k++) {
+    lu_pivot(a, fac, k, tag, n);
+    for (int i = k + 1; i < n; i++) {
+      double f = -a[i * n + k] / a[k * n + k];
+      fac[i * n + k] = -f;
+      for (int j = k; j <= n - 1; j++) a[i * n + j] = f * a[k * n + j] + a[i * n + j];
+    }
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      if (i == j) fac[i * n + j] = 1;
+      if (j > i) fac[i * n + j] = 0;
+    }
+  /* columns of the inverse by forward then backward substitution on e_i */
+  for (int col = 0; col < n; col++) {
+    double d[4];
+    for (int i = 0; i < n; i++) d[i] = (i == col) ? 1.0 : 0.0;
+    vec1[0] = d[0];
+    for (int i = 1; i < n; i++) {
+      double s = 0;
+      for (int j = 0; j < i; j++) s = s - fac[i * n + j] * vec1[j];
+      vec1[i] = d[i] + s;
+    }
+    vec[n - 1] = vec1[n - 1] * pow(a[(n - 1) * n + (n - 1)], -1);
+    for (int i = n - 2; i >= 0; i--) {
+      double s = 0;
+      for (int j = i + 1; j < n; j++) s = s - a[i * n + j] * vec[j];
+      vec[i] = (vec1[i] + s) * pow(a[i * n + i], -1);
+    }
+    for (int j = 0; j < n; j++) inv1[j * n + col] = vec[j];
+  }
+  /* undo the column permutation of the inverse */
+  for (int j = 0; j < n; j++)
+    for (int p = 0; p < n; p++) inv[p * n + tag[j]] = inv1[p * n + j];
+  /* the input is overwritten by the row-permuted product L*U */
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < n; i++) {
+      double s = 0;
+      for (int j = 0; j < n; j++) s = s + fac[k * n + j] * a[j * n + i];
+      prod[k * n + i] = s;
+    }
+  for (int j = 0; j < n; j++)
+    for (int p = 0; p < n; p++) a[tag[j] * n + p] = prod[j * n + p];
+}
+
+static void mat3_mul(const double m1[3][3], const double m2[3][3], double out[3][3]) {
+  for (int k = 0; k < 3; k++)
+    for (int i = 0; i < 3; i++) {
+      double s = 0;
+      for (int j = 0; j < 3; j++) s = s + m1[k][j] * m2[j][i];
+      out[k][i] = s;
+    }
+}
+
+/* utility_functions.h:655-699 */
+static void rotation_about(double R[3][3], double theta, int axis) {
+  double c = cos(theta * M_PI / 180.0), s = sin(theta * M_PI / 180.0);
+  if (axis == 0) {
+    R[0][0] = 1.0; R[0][1] = 0.0; R[0][2] = 0.0; R[1][0] = 0.0; R[2][0] = 0.0;
+    R[1][1] = c;   R[1][2] = -s;  R[2][1] = s;   R[2][2] = c;
+  } else if (axis == 1) {
+    R[0][0] = c;   R[0][1] = 0.0; R[0][2] = s;   R[1][0] = 0.0; R[2][0] = -s;
+    R[1][1] = 1.0; R[1][2] = 0.0; R[2][1] = 0.0; R[2][2] = c;
+  } else {
+    R[0][0] = c;   R[0][1] = -s;  R[0][2] = 0.0; R[1][0] = s;   R[2][0] = 0.0;
+    R[1][1] = c;   R[1][2] = 0.0; R[2][1] = 0.0; R[2][2] = 1.0;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Thermodynamic constants (functionF_01.h)                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct derive_tmp {
+  double DT[P_MAX][P_MAX];  /* DELTA_T */
+  double DC[P_MAX][C_MAX];  /* DELTA_C */
+} derive_tmp;
+
+static double fn_B(const gp_params *p, const derive_tmp *t, double T, int i, int a) {
+  int P = p->nphases, C = p->ncomp, L = P - 1;
+  double cl[C_MAX], cs[C_MAX], sum_c = 0.0, B = 0.0;
+  if (a != L) {
+    for (int k = 0; k < C; k++) {
+      cl[k] = p->ceq[a][L][k] - (t->DC[a][k]) * (p->Teq - T) / (t->DT[a][L]);
+      cs[k] = p->ceq[a][a][k] - (t->DC[a][k]) * (p->Teq - T) / (t->DT[a][a]);
+      if (k != i) sum_c += p->A[L][k][i] * cl[k] - p->A[a][k][i] * cs[k];
+    }
+    B = (2.0 * (p->A[L][i][i] * cl[i] - p->A[a][i][i] * cs[i]) + sum_c);
+  }
+  return B;
+}
+
+static double fn_C(const gp_params *p, const derive_tmp *t, double T, int a) {
+  int P = p->nphases, C = p->ncomp, L = P - 1;
+  double cl[C_MAX], cs[C_MAX], sum_c = 0.0;
+  if (a != L) {
+    for (int k = 0; k < C; k++) {
+      cl[k] = p->ceq[a][L][k] - (t->DC[a][k]) * (p->Teq - T) / (t->DT[a][L]);
+      cs[k] = p->ceq[a][a][k] - (t->DC[a][k]) * (p->Teq - T) / (t->DT[a][a]);
+    }
+    for (int i = 0; i < C; i++)
+      for (int j = 0; j < C; j++)
+        if (i <= j) sum_c += (p->A[a][i][j] * cs[i] * cs[j] - p->A[L][i][j] * cl[i] * cl[j]);
+  }
+  return sum_c;
+}
+
+int gp_derive(gp_params *p) {
+  int P = p->nphases, C = p->ncomp, L = P - 1;
+  if (P < 2 || P > P_MAX || C < 1 || C > C_MAX) return GP_E_ARG;
+  derive_tmp t;
+  memset(&t, 0, sizeof(t));
+  /* cmu = (d mu / d c)^-1 with the reference's LU (initialize_variables.h:6-17) */
+  for (int a = 0; a < P; a++) {
+    double muc[16], inv[16];
+    for (int i = 0; i < C; i++)
+      for (int j = 0; j < C; j++) muc[i * C + j] = (i == j) ? 2.0 * p->A[a][i][j] : p->A[a][i][j];
+    gp_matinv(muc, inv, C);
+    for (int i = 0; i < C; i++)
+      for (int j = 0; j < C; j++) p->cmu[a][i][j] = inv[i * C + j];
+  }
+  for (int a = 0; a < L; a++) {
+    double dcdT_s[C_MAX], dcdT_l[C_MAX];
+    t.DT[a][L] = 0.0;
+    t.DT[a][a] = 0.0;
+    for (int k = 0; k < C; k++) {
+      t.DT[a][a] += p->slopes[a][a][k] * (p->ceq[a][L][k] - p->ceq[a][a][k]);
+      t.DT[a][L] += p->slopes[a][L][k] * (p->ceq[a][L][k] - p->ceq[a][a][k]);
+      t.DC[a][k] = p->ceq[a][L][k] - p->ceq[a][a][k];
+    }
+    for (int k = 0; k < C; k++) {
+      dcdT_s[k] = t.DC[a][k] / t.DT[a][a];
+      dcdT_l[k] = t.DC[a][k] / t.DT[a][L];
+    }
+    for (int k = 0; k < C; k++) {
+      p->dBbdT[a][k] = 2.0 * (p->A[L][k][k] * dcdT_l[k] - p->A[a][k][k] * dcdT_s[k]);
+      for (int i = 0; i < C; i++)
+        if (k != i) p->dBbdT[a][k] += (p->A[L][k][i] * dcdT_l[i] - p->A[a][k][i] * dcdT_s[i]);
+    }
+  }
+  for (int k = 0; k < C; k++) p->dBbdT[L][k] = 0.0;
+  for (int a = 0; a < P; a++)
+    for (int i = 0; i < C; i++) p->Beq[a][i] = fn_B(p, &t, p->Teq, i, a);
+  for (int a = 0; a < P; a++) {
+    for (int i = 0; i < C; i++) p->B[a][i] = fn_B(p, &t, p->T, i, a);
+    p->Cf[a] = fn_C(p, &t, p->T, a);
+  }
+  return GP_OK;
+}
+
+double gp_mu_of_c(const gp_params *p, const double *c, double T, int a, int i) {
+  double s = 2.0 * p->A[a][i][i] * c[i] + (p->Beq[a][i] + p->dBbdT[a][i] * (T - p->Teq));
+  for (int j = 0; j < p->ncomp; j++)
+    if (i != j) s += p->A[a][i][j] * c[j];
+  return s;
+}
+
+static double c_of_mu(const gp_params *p, const double *mu, double T, int a, int i) {
+  double s = 0.0;
+  for (int k = 0; k < p->ncomp; k++)
+    s += p->cmu[a][i][k] * (mu[k] - (p->Beq[a][k] + p->dBbdT[a][k] * (T - p->Teq)));
+  return s;
+}
+
+static double h_of_phi(const double *phi, int P, int a) {
+  double s1 = 0.0;
+  double s = 3.0 * phi[a] * phi[a] - 2.0 * phi[a] * phi[a] * phi[a];
+  for (int b = 0; b < P; b++)
+    for (int c = 0; c < P; c++)
+      if (b != a && c != a && b < c) s1 += phi[b] * phi[c];
+  s1 *= 2.0 * phi[a];
+  return s + s1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Input.in                                                                  */
+/* ------------------------------------------------------------------------ */
+
+#define MAX_TOK 600
+typedef struct toks {
+  int n;
+  char s[MAX_TOK][40];
+} toks;
+
+/* strtok_r(value, "{,}") as the populate_* helpers do; tokens keep blanks */
+static void tokenize(const char *v, toks *t) {
+  char buf[GP_VAL_LEN];
+  char *save = NULL, *tok;
+  snprintf(buf, sizeof(buf), "%s", v);
+  t->n = 0;
+  for (char *s = buf;; s = NULL) {
+    tok = strtok_r(s, "{,}", &save);
+    if (!tok) break;
+    if (t->n < MAX_TOK) snprintf(t->s[t->n++], 40, "%s", tok);
+  }
+}
+
+static int pair_matrix(double m[P_MAX][P_MAX], const char *v, int P) {
+  toks t;
+  tokenize(v, &t);
+  int need = P * (P - 1) / 2, k = 0;
+  if (t.n < need) return -1;
+  for (int i = 0; i < P; i++)
+    for (int j = i + 1; j < P; j++) {
+      m[i][i] = 0.0;
+      m[i][j] = atof(t.s[k]);
+      m[j][i] = m[i][j];
+      k++;
+    }
+  return 0;
+}
+
+static int triple_matrix(double m[P_MAX][P_MAX][P_MAX], const char *v, int P) {
+  toks t;
+  tokenize(v, &t);
+  int need = P * (P - 1) * (P - 2) / 6, l = 0;
+  if (t.n < need) return -1;
+  for (int i = 0; i < P; i++)
+    for (int j = i + 1; j < P; j++)
+      for (int k = j + 1; k < P; k++) {
+        m[i][i][i] = 0.0; m[i][j][j] = 0.0; m[i][k][k] = 0.0;
+        double x = atof(t.s[l++]);
+        m[i][j][k] = x; m[i][k][j] = x; m[j][i][k] = x;
+        m[j][k][i] = x; m[k][i][j] = x; m[k][j][i] = x;
+      }
+  return 0;
+}
+
+static int valid_phase(long ph, int P) { return ph >= 0 && ph < P; }
+
+int gp_read_input(const char *path, gp_input *in, int verbose, char *err, size_t errlen) {
+  FILE *f = fopen(path, "rt");
+  if (!f) {
+    set_err(err, errlen, "file %s not found", path);
+    return GP_E_IO;
+  }
+  memset(in, 0, sizeof(*in));
+  gp_params *p = &in->p;
+  in->function_w = 1;
+  in->function_anisotropy = 1;
+  in->function_f = 1;
+  in->dimension = 2;
+  char line[GP_LINE_LEN], key[128] = "", val[GP_VAL_LEN] = "";
+  int P = 0, C1 = 0; /* NUMPHASES, NUMCOMPONENTS as read so far */
+  int rc = GP_OK;
+  char msg[200] = "";
+
+  /* first pass: reading_input_parameters() */
+  while (fgets(line, sizeof(line), f)) {
+    /* a line that does not match leaves key/val as they were, so a blank line
+       re-applies the previous key, like the reference's sscanf */
+    sscanf(line, "%100s = %" GP_VAL_FMT "[^;];", key, val);
+    if (key[0] == '#') continue;
+    int haveP = P > 0, haveC = (C1 - 1) > 0;
+    if (!strcmp(key, "DIMENSION")) in->dimension = atoi(val);
+    else if (!strcmp(key, "MESH_X")) p->mesh_x = atol(val);
+    else if (!strcmp(key, "MESH_Y")) p->mesh_y = atol(val);
+    else if (!strcmp(key, "MESH_Z")) in->mesh_z = atol(val);
+    else if (!strcmp(key, "DELTA_X")) p->dx = atof(val);
+    else if (!strcmp(key, "DELTA_Y")) p->dy = atof(val);
+    else if (!strcmp(key, "DELTA_Z")) in->dz = atof(val);
+    else if (!strcmp(key, "DELTA_t")) p->dt = atof(val);
+    else if (!strcmp(key, "NUMPHASES")) {
+      P = atoi(val);
+      if (P < 1 || P > P_MAX) { snprintf(msg, sizeof msg, "NUMPHASES=%d outside [2,%d]", P, P_MAX); rc = GP_E_UNSUPPORTED; P = P < 1 ? 0 : P_MAX; }
+      p->nphases = P;
+    }
+    else if (!strcmp(key, "NUMCOMPONENTS")) {
+      C1 = atoi(val);
+      if (C1 < 2 || C1 - 1 > C_MAX) { snprintf(msg, sizeof msg, "NUMCOMPONENTS=%d outside [2,%d]", C1, C_MAX + 1); rc = GP_E_UNSUPPORTED; C1 = C1 < 2 ? 1 : C_MAX + 1; }
+      p->ncomp = C1 - 1;
+    }
+    else if (!strcmp(key, "epsilon")) p->epsilon = atof(val);
+    else if (!strcmp(key, "tau")) p->tau = atof(val);
+    else if (!strcmp(key, "NTIMESTEPS")) in->ntimesteps = atol(val);
+    else if (!strcmp(key, "SAVET")) in->saveT = atol(val);
+    else if (!strcmp(key, "NSMOOTH")) in->nsmooth = atol(val);
+    else if (!strcmp(key, "STARTTIME")) in->starttime = atol(val);
+    else if (!strcmp(key, "RESTART")) in->restart = atol(val);
+    else if (!strcmp(key, "R")) in->R = atof(val);
+    else if (!strcmp(key, "V")) in->V = atof(val);
+    else if (!strcmp(key, "WRITEFORMAT")) {
+      if (!strcmp(val, "ASCII")) in->ascii = 1;
+      else if (!strcmp(val, "BINARY")) in->ascii = 0;
+    }
+    else if (!strcmp(key, "TRACK_PROGRESS")) in->track_progress = atol(val);
+    else if (!strcmp(key, "TEMPERATURE_SCALE")) in->temperature_scale = atof(val);
+    else if (!strcmp(key, "Equilibrium_temperature")) p->Teq = atof(val);
+    else if (!strcmp(key, "Filling_temperature")) in->Tfill = atof(val);
+    else if (!strcmp(key, "T")) p->T = atof(val);
+    else if (!strcmp(key, "tilt_angle")) { /* read, unused by the step */ }
+    else if (!strcmp(key, "Rtheta")) { /* read, unused by the step */ }
+    else if (!strcmp(key, "COMPONENTS")) {
+      toks t; tokenize(val, &t);
+      for (int i = 0; i < t.n && i < C_MAX + 1; i++) snprintf(in->components[i], GP_NAME_LEN, "%s", t.s[i]);
+      in->ncomponents_names = t.n;
+    }
+    else if (!strcmp(key, "PHASES")) {
+      toks t; tokenize(val, &t);
+      for (int i = 0; i < t.n && i < P_MAX; i++) snprintf(in->phases[i], GP_NAME_LEN, "%s", t.s[i]);
+      in->nphases_names = t.n;
+    }
+    else if (!strcmp(key, "Function_anisotropy") && haveP && haveC) in->function_anisotropy = atoi(val);
+    else if (!strcmp(key, "Anisotropy_type") && in->function_anisotropy != 0) in->fold = atoi(val);
+    else if (!strcmp(key, "Rotation_matrix") && in->function_anisotropy != 0) {
+      toks t; tokenize(val, &t);
+      if (t.n < 5) { snprintf(msg, sizeof msg, "Rotation_matrix needs 5 entries"); rc = GP_E_ARG; continue; }
+      long p1 = atol(t.s[0]), p2 = atol(t.s[1]);
+      if (!valid_phase(p1, P) || !valid_phase(p2, P)) { snprintf(msg, sizeof msg, "Rotation_matrix phase index out of range"); rc = GP_E_ARG; continue; }
+      double Rx[3][3], Ry[3][3], Rz[3][3], m[3][3], R[3][3];
+      rotation_about(Rx, atof(t.s[2]), 0);
+      rotation_about(Ry, atof(t.s[3]), 1);
+      rotation_about(Rz, atof(t.s[4]), 2);
+      if (verbose) printf("phase1=%ld, phase2=%ld\n", p1, p2);
+      mat3_mul(Rx, Ry, m);
+      mat3_mul(m, Rz, R);
+      double a9[9], i9[9];
+      for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) a9[i * 3 + j] = R[i][j];
+      gp_matinv(a9, i9, 3);
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          p->rot[p1][p2][i][j] = p->rot[p2][p1][i][j] = a9[i * 3 + j];
+          p->rot_inv[p1][p2][i][j] = p->rot_inv[p2][p1][i][j] = i9[i * 3 + j];
+        }
+      in->rot_given[p1][p2] = in->rot_given[p2][p1] = 1;
+    }
+    else if (!strcmp(key, "OBSTACLE")) { /* overridden by Function_W (initialize_functions_solverloop.h:13-23) */ }
+    else if (!strcmp(key, "Function_W") && haveP && haveC) in->function_w = atoi(val);
+    else if (!strcmp(key, "ISOTHERMAL")) {
+      p->isothermal = atoi(val);
+      if (!p->isothermal) in->tempgrady = 1;
+    }
+    else if (!strcmp(key, "Tempgrady") && in->tempgrady) {
+      toks t; tokenize(val, &t);
+      for (int i = 0; i < 5 && i < t.n; i++) in->tempgrad[i] = atof(t.s[i]);
+    }
+    else if (!strcmp(key, "Shift")) in->shift = atoi(val);
+    else if (!strcmp(key, "Shiftj")) { if (in->shift) in->shiftj = atol(val); }
+    else if (!strcmp(key, "BINARY")) { p->mu_solver = atoi(val) ? GP_MU_BINARY : GP_MU_GENERAL; }
+    else if (!strcmp(key, "TERNARY")) { p->mu_solver = atoi(val) ? GP_MU_TERNARY : GP_MU_GENERAL; }
+    else if (!strcmp(key, "DILUTE")) { p->mu_solver = atoi(val) ? GP_MU_DILUTE : GP_MU_GENERAL; }
+    else if (!strcmp(key, "Writecomposition")) in->writecomposition = atoi(val);
+    else if (!strcmp(key, "Noise_phasefield")) in->noise_phasefield = atoi(val);
+    else if (!strcmp(key, "Amp_Noise_Phase")) { if (in->noise_phasefield) in->amp_noise_phase = atof(val); }
+    else if (!strcmp(key, "GAMMA") && haveP) { if (pair_matrix(p->gamma, val, P)) { snprintf(msg, sizeof msg, "GAMMA needs P(P-1)/2 entries"); rc = GP_E_ARG; } else in->have_gamma = 1; }
+    else if (!strcmp(key, "Tau") && haveP) { if (pair_matrix(p->tau_ab, val, P)) { snprintf(msg, sizeof msg, "Tau needs P(P-1)/2 entries"); rc = GP_E_ARG; } else in->have_tau = 1; }
+    else if (!strcmp(key, "dab") && haveP) { if (pair_matrix(p->dab, val, P)) { snprintf(msg, sizeof msg, "dab needs P(P-1)/2 entries"); rc = GP_E_ARG; } else in->have_dab = 1; }
+    else if ((!strcmp(key, "ec") || !strcmp(key, "e2") || !strcmp(key, "e4") || !strcmp(key, "beta")) && haveP) { /* other anisotropy forms: parsed by the reference, unused by the 4-fold step */ }
+    else if (!strcmp(key, "Gamma_abc") && haveP) { if (triple_matrix(p->gamma_abc, val, P)) { snprintf(msg, sizeof msg, "Gamma_abc needs P(P-1)(P-2)/6 entries"); rc = GP_E_ARG; } else in->have_gamma_abc = 1; }
+    else if (!strcmp(key, "DIFFUSIVITY") && haveP && haveC) {
+      toks t; tokenize(val, &t);
+      int C = C1 - 1;
+      if (t.n < 2 + C) { snprintf(msg, sizeof msg, "DIFFUSIVITY entry too short"); rc = GP_E_ARG; continue; }
+      long ph = atol(t.s[1]);
+      if (!valid_phase(ph, P)) { snprintf(msg, sizeof msg, "DIFFUSIVITY phase out of range"); rc = GP_E_ARG; continue; }
+      int l = 1;
+      for (int i = 0; i < C; i++) { p->diffusivity[ph][i][i] = atof(t.s[1 + l]); l++; }
+      if (!strcmp(t.s[0], "1")) {
+        for (int i = 0; i < C; i++) for (int j = i + 1; j < C; j++) { p->diffusivity[ph][i][j] = 0.0; p->diffusivity[ph][j][i] = 0.0; }
+      } else {
+        if (t.n < 2 + C + C * C) { snprintf(msg, sizeof msg, "full DIFFUSIVITY entry too short"); rc = GP_E_ARG; continue; }
+        for (int i = 0; i < C; i++) for (int j = 0; j < C; j++) { p->diffusivity[ph][i][j] = atof(t.s[1 + l]); l++; }
+      }
+    }
+    else if ((!strcmp(key, "ceq") || !strcmp(key, "cfill") || !strcmp(key, "slopes")) && haveP && haveC) {
+      toks t; tokenize(val, &t);
+      int C = C1 - 1;
+      if (t.n < 2 + C) { snprintf(msg, sizeof msg, "%s entry too short", key); rc = GP_E_ARG; continue; }
+      long p1 = atoi(t.s[0]), p2 = atoi(t.s[1]);
+      if (!valid_phase(p1, P) || !valid_phase(p2, P)) { snprintf(msg, sizeof msg, "%s phase out of range", key); rc = GP_E_ARG; continue; }
+      double (*m)[P_MAX][C_MAX] = !strcmp(key, "ceq") ? p->ceq : !strcmp(key, "cfill") ? p->cfill : p->slopes;
+      for (int i = 0; i < C; i++) m[p1][p2][i] = atof(t.s[2 + i]);
+    }
+    else if (!strcmp(key, "Function_F") && haveP && haveC) in->function_f = atoi(val);
+    else if (!strcmp(key, "A") && haveP && haveC) {
+      toks t; tokenize(val, &t);
+      int C = C1 - 1, l = 1;
+      if (t.n < 1 + C + C * (C - 1) / 2) { snprintf(msg, sizeof msg, "A entry too short"); rc = GP_E_ARG; continue; }
+      long ph = atol(t.s[0]);
+      if (!valid_phase(ph, P)) { snprintf(msg, sizeof msg, "A phase out of range"); rc = GP_E_ARG; continue; }
+      for (int i = 0; i < C; i++) p->A[ph][i][i] = atof(t.s[l++]);
+      for (int i = 0; i < C; i++) for (int j = i + 1; j < C; j++) { p->A[ph][i][j] = atof(t.s[l++]); p->A[ph][j][i] = p->A[ph][i][j]; }
+    }
+    else if (!strcmp(key, "EIGEN_STRAIN") && haveP) { /* elasticity keys: read, unused by the step */ }
+    else if (!strcmp(key, "VOIGT_ISOTROPIC") && haveP) { }
+    else {
+      in->unknown_keys++;
+      if (verbose) printf("Unrecongized parameter : \"%s\"\n", key);
+    }
+  }
+
+  /* second pass: read_boundary_conditions() */
+  rewind(f);
+  key[0] = val[0] = '\0';
+  long rows[2] = {p->mesh_x + 6, p->mesh_y + 6};
+  while (fgets(line, sizeof(line), f)) {
+    sscanf(line, "%100s = %" GP_VAL_FMT "[^;];", key, val);
+    if (key[0] == '#' || P <= 0) continue;
+    if (!strcmp(key, "BOUNDARY")) {
+      toks t; tokenize(val, &t);
+      if (t.n < 7) { snprintf(msg, sizeof msg, "BOUNDARY needs 7 entries"); rc = GP_E_ARG; continue; }
+      int fld = !strcmp(t.s[0], "phi") ? 0 : !strcmp(t.s[0], "mu") ? 1 : !strcmp(t.s[0], "T") ? 2 : -1;
+      if (fld < 0) continue;
+      int ty[4];
+      ty[GP_FACE_XMIN] = atoi(t.s[2]);
+      ty[GP_FACE_XMAX] = atoi(t.s[1]);
+      ty[GP_FACE_YMAX] = atoi(t.s[3]);
+      ty[GP_FACE_YMIN] = atoi(t.s[4]);
+      if (ty[0] == 3 || ty[1] == 3) ty[0] = ty[1] = 3;   /* periodic on one side forces both */
+      if (ty[2] == 3 || ty[3] == 3) ty[2] = ty[3] = 3;
+      for (int fc = 0; fc < 4; fc++) {
+        if (fld < 2) p->bc_type[fld][fc] = ty[fc];
+        else in->bc_type_T[fc] = ty[fc];
+      }
+    } else if (!strcmp(key, "BOUNDARY_VALUE")) {
+      toks t; tokenize(val, &t);
+      if (t.n < 7) { snprintf(msg, sizeof msg, "BOUNDARY_VALUE needs 7 entries"); rc = GP_E_ARG; continue; }
+      int fld = !strcmp(t.s[0], "phi") ? 0 : !strcmp(t.s[0], "mu") ? 1 : !strcmp(t.s[0], "T") ? 2 : -1;
+      if (fld < 0) continue;
+      in->bc_value[fld][GP_FACE_XMIN] = atof(t.s[2]);
+      in->bc_value[fld][GP_FACE_XMAX] = atof(t.s[1]);
+      in->bc_value[fld][GP_FACE_YMAX] = atof(t.s[3]);
+      in->bc_value[fld][GP_FACE_YMIN] = atof(t.s[4]);
+      if (fld == 2) continue;
+      /* proxies follow the face type at this point of the file */
+      for (int fc = 0; fc < 4; fc++) {
+        int ty = p->bc_type[fld][fc];
+        long n = (fc == GP_FACE_XMIN || fc == GP_FACE_XMAX) ? rows[0] : rows[1];
+        int low = (fc == GP_FACE_XMIN || fc == GP_FACE_YMIN);
+        if (ty == GP_BC_NEUMANN || ty == GP_BC_PERIODIC) {
+          int mirror_low = (ty == GP_BC_NEUMANN) == low;   /* proxies 4,5,6 */
+          for (int j = 0; j < 3; j++) p->bc_proxy[fld][fc][j] = mirror_low ? 4 + j : n - 5 - j;
+          in->bc_proxy_set[fld][fc] = 1;
+        }
+      }
+    }
+  }
+  fclose(f);
+
+  /* model switches (initialize_functions_solverloop.h:4-58) */
+  p->obstacle = (in->function_w == 1);
+  p->anisotropy = in->function_anisotropy;
+  if (rc != GP_OK) set_err(err, errlen, "%s", msg);
+  return rc;
+}
+
+int gp_check_supported(const gp_params *p, char *err, size_t errlen) {
+  char m[160];
+  if (p->nphases < 2 || p->nphases > P_MAX || p->ncomp < 1 || p->ncomp > C_MAX) { set_err(err, errlen, "%s", "NUMPHASES/NUMCOMPONENTS out of range"); return GP_E_UNSUPPORTED; }
+  if (p->mesh_x < 4 || p->mesh_y < 4) { set_err(err, errlen, "%s", "MESH_X and MESH_Y must be >= 4"); return GP_E_ARG; }
+  if (!(p->dx > 0) || !(p->dy > 0) || !(p->dt > 0) || !(p->epsilon != 0)) { set_err(err, errlen, "%s", "DELTA_X, DELTA_Y, DELTA_t must be > 0 and epsilon != 0"); return GP_E_ARG; }
+  if (!p->isothermal) { set_err(err, errlen, "%s", "non-isothermal (Tempgrady) runs are not stepped by this build"); return GP_E_UNSUPPORTED; }
+  if (p->anisotropy != 0 && p->anisotropy != 1) { set_err(err, errlen, "%s", "Function_anisotropy must be 0 or 1"); return GP_E_UNSUPPORTED; }
+  if (p->mu_solver < 0 || p->mu_solver > 3) { set_err(err, errlen, "%s", "bad mu_solver"); return GP_E_ARG; }
+  if (p->mu_solver == GP_MU_BINARY && p->ncomp != 1) { /* the reference updates only mu[0]; keep that */ }
+  if (p->mu_solver == GP_MU_TERNARY && p->ncomp != 2) { set_err(err, errlen, "%s", "TERNARY needs NUMCOMPONENTS = 3"); return GP_E_UNSUPPORTED; }
+  for (int f = 0; f < 2; f++)
+    for (int fc = 0; fc < 4; fc++) {
+      int ty = p->bc_type[f][fc];
+      long n = (fc < 2) ? p->mesh_x : p->mesh_y;
+      if (ty != GP_BC_NEUMANN && ty != GP_BC_PERIODIC) {
+        snprintf(m, sizeof m, "boundary type %d on %s face %d: only Neumann (1) and periodic (3) ghost copies are stepped", ty, f ? "mu" : "phi", fc);
+        set_err(err, errlen, "%s", m);
+        return GP_E_UNSUPPORTED;
+      }
+      for (int j = 0; j < 3; j++) {
+        long q = p->bc_proxy[f][fc][j];
+        if (q < 3 || q > n + 2) { set_err(err, errlen, "%s", "boundary proxy outside the interior (BOUNDARY_VALUE missing?)"); return GP_E_ARG; }
+      }
+    }
+  return GP_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Filling.in                                                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct padgrid {
+  long rx, ry;
+  int P, C;
+  double *phi; /* [P][rx][ry] */
+  double *mu;  /* [C][rx][ry] */
+} padgrid;
+
+#define PH(g, a, x, y) (g)->phi[((size_t)(a) * (g)->rx + (x)) * (g)->ry + (y)]
+#define MU(g, k, x, y) (g)->mu[((size_t)(k) * (g)->rx + (x)) * (g)->ry + (y)]
+
+/* set phase b inside the region, clear b elsewhere unless already 1; the liquid
+   call completes the simplex (filling.h:6-52,117-168) */
+static void fill_region(padgrid *g, int b, int (*inside)(const void *, long, long), const void *arg, int verbose) {
+  int P = g->P;
+  for (long x = 0; x < g->rx; x++)
+    for (long y = 0; y < g->ry; y++) {
+      if (b < P - 1) {
+        if (inside(arg, x, y)) {
+          PH(g, b, x, y) = 1.0;
+          for (int a = 0; a < P; a++) if (a != b) PH(g, a, x, y) = 0.0;
+        } else if (PH(g, b, x, y) != 1.0) {
+          PH(g, b, x, y) = 0.0;
+        }
+      } else {
+        double sum = 0.0;
+        for (int a = 0; a < P - 1; a++) sum += PH(g, a, x, y);
+        if (sum > 1.0) {
+          if (verbose) printf("Wrong filling operation, will fill it with liquid\n");
+          PH(g, b, x, y) = 1.0;
+          for (int a = 0; a < P - 1; a++) if (a != b) PH(g, a, x, y) = 0.0;
+        } else {
+          PH(g, b, x, y) = 1.0 - sum;
+        }
+      }
+    }
+}
+
+typedef struct { long x0, x1, y0, y1, z0, z1; } cube_t;
+static int in_cube(const void *v, long x, long y) {
+  const cube_t *c = (const cube_t *)v;
+  long z = 0;
+  return x >= c->x0 && x <= c->x1 && y >= c->y0 && y <= c->y1 && z >= c->z0 && z <= c->z1;
+}
+typedef struct { long xc, yc, z0, z1; double r; } cyl_t;
+static int in_cyl(const void *v, long x, long y) {
+  const cyl_t *c = (const cyl_t *)v;
+  long z = 0;
+  return ((x - c->xc) * (x - c->xc) + (y - c->yc) * (y - c->yc) <= c->r * c->r) && z >= c->z0 && z <= c->z1;
+}
+typedef struct { long xc, yc, zc; double r; } sph_t;
+static int in_sph(const void *v, long x, long y) {
+  const sph_t *c = (const sph_t *)v;
+  long z = 0;
+  return (x - c->xc) * (x - c->xc) + (y - c->yc) * (y - c->yc) + (z - c->zc) * (z - c->zc) <= c->r * c->r;
+}
+typedef struct { long xc, yc; double l, e, ang; } ell_t;
+static int in_ell(const void *v, long x, long y) {
+  const ell_t *c = (const ell_t *)v;
+  double th = c->ang * M_PI / 180.0;
+  double u = (double)(x - c->xc) * cos(th) + (double)(y - c->yc) * sin(th);
+  double w = -(double)(x - c->xc) * sin(th) + (double)(y - c->yc) * cos(th);
+  return u * u / (c->l * c->l) + w * w / (c->e * c->e * c->l * c->l) <= 1.0;
+}
+
+int gp_fill_domain(const gp_input *in, const char *filling_path, double *phi, double *mu,
+                   int verbose, char *err, size_t errlen) {
+  const gp_params *p = &in->p;
+  padgrid g;
+  g.P = p->nphases;
+  g.C = p->ncomp;
+  g.rx = p->mesh_x + 6;
+  g.ry = p->mesh_y + 6;
+  size_t n = (size_t)g.rx * g.ry;
+  g.phi = (double *)calloc(n * g.P, sizeof(double));
+  g.mu = (double *)calloc(n * g.C, sizeof(double));
+  if (!g.phi || !g.mu) { free(g.phi); free(g.mu); return GP_E_NOMEM; }
+  FILE *f = fopen(filling_path, "rt");
+  if (!f) {
+    set_err(err, errlen, "file %s not found", filling_path);
+    free(g.phi); free(g.mu);
+    return GP_E_IO;
+  }
+  char line[GP_LINE_LEN], key[128] = "", val[GP_VAL_LEN] = "";
+  int P = g.P;
+  while (fgets(line, sizeof(line), f)) {
+    sscanf(line, "%100s = %" GP_VAL_FMT "[^;];", key, val);
+    if (key[0] == '#') continue;
+    toks t;
+    if (!strcmp(key, "FILLCUBE")) {
+      tokenize(val, &t);
+      if (t.n < 7) continue;
+      long ph = atol(t.s[0]);
+      cube_t c = {atol(t.s[1]) + 3, atol(t.s[4]) + 3, atol(t.s[2]) + 3, atol(t.s[5]) + 3, atol(t.s[3]), atol(t.s[6])};
+      if (!valid_phase(ph, P)) continue;
+      fill_region(&g, (int)ph, in_cube, &c, verbose);
+      fill_region(&g, P - 1, in_cube, &c, verbose);
+    } else if (!strcmp(key, "FILLCYLINDER")) {
+      if (verbose) printf("Filling cylinder\n");
+      tokenize(val, &t);
+      if (t.n < 6) continue;
+      long ph = atol(t.s[0]);
+      cyl_t c = {atol(t.s[1]) + 3, atol(t.s[2]) + 3, atol(t.s[3]), atol(t.s[4]), atof(t.s[5])};
+      if (!valid_phase(ph, P)) continue;
+      fill_region(&g, (int)ph, in_cyl, &c, verbose);
+      fill_region(&g, P - 1, in_cyl, &c, verbose);
+      if (verbose) printf("End filling cylinder\n");
+    } else if (!strcmp(key, "FILLSPHERE")) {
+      tokenize(val, &t);
+      if (t.n < 5) continue;
+      long ph = atol(t.s[0]);
+      sph_t c = {atol(t.s[1]) + 3, atol(t.s[2]) + 3, atol(t.s[3]), atof(t.s[4])};
+      if (!valid_phase(ph, P)) continue;
+      fill_region(&g, (int)ph, in_sph, &c, verbose);
+      fill_region(&g, P - 1, in_sph, &c, verbose);
+    } else if (!strcmp(key, "FILLELLIPSE")) {
+      tokenize(val, &t);
+      if (t.n < 7) continue;
+      long ph = atol(t.s[0]);
+      ell_t c = {atol(t.s[1]) + 3, atol(t.s[2]) + 3, atof(t.s[4]), atof(t.s[5]), atof(t.s[6])};
+      if (!valid_phase(ph, P)) continue;
+      fill_region(&g, (int)ph, in_ell, &c, verbose);
+      fill_region(&g, P - 1, in_ell, &c, verbose);
+    }
+  }
+  fclose(f);
+  if (verbose) printf("Filling composition\n");
+  /* fill_composition_cube (filling.h:221-270): solid cells at their equilibrium
+     composition, the rest at the liquid filling composition, both at Teq */
+  for (long x = 0; x < g.rx; x++)
+    for (long y = 0; y < g.ry; y++) {
+      double c[C_MAX];
+      int filled = 0;
+      for (int b = 0; b < P - 1; b++) {
+        if (PH(&g, b, x, y) == 1.0) {
+          for (int k = 0; k < g.C; k++) c[k] = p->ceq[b][b][k];
+          for (int k = 0; k < g.C; k++) MU(&g, k, x, y) = gp_mu_of_c(p, c, p->Teq, b, k);
+          filled = 1;
+          break;
+        }
+      }
+      if (!filled) {
+        for (int k = 0; k < g.C; k++) c[k] = p->cfill[P - 1][P - 1][k];
+        for (int k = 0; k < g.C; k++) MU(&g, k, x, y) = gp_mu_of_c(p, c, p->Teq, P - 1, k);
+      }
+    }
+  for (int a = 0; a < P; a++)
+    for (long x = 0; x < p->mesh_x; x++)
+      for (long y = 0; y < p->mesh_y; y++)
+        phi[((size_t)a * p->mesh_x + x) * p->mesh_y + y] = PH(&g, a, x + 3, y + 3);
+  for (int k = 0; k < g.C; k++)
+    for (long x = 0; x < p->mesh_x; x++)
+      for (long y = 0; y < p->mesh_y; y++)
+        mu[((size_t)k * p->mesh_x + x) * p->mesh_y + y] = MU(&g, k, x + 3, y + 3);
+  free(g.phi);
+  free(g.mu);
+  return GP_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Legacy VTK                                                                */
+/* ------------------------------------------------------------------------ */
+
+static double to_big_endian(double v) {
+  uint64_t u;
+  memcpy(&u, &v, 8);
+  u = __builtin_bswap64(u);
+  memcpy(&v, &u, 8);
+  return v;
+}
+
+int gp_vtk_write(const char *path, const gp_input *in, const double *phi, const double *mu, int binary) {
+  const gp_params *p = &in->p;
+  FILE *f = fopen(path, binary ? "wb" : "w");
+  if (!f) return GP_E_IO;
+  long MX = p->mesh_x, MY = p->mesh_y;
+  size_t n = (size_t)MX * MY;
+  fprintf(f, "# vtk DataFile Version 3.0\n");
+  fprintf(f, "Microsim_fields\n");
+  fprintf(f, binary ? "BINARY\n" : "ASCII\n");
+  fprintf(f, "DATASET STRUCTURED_POINTS\n");
+  fprintf(f, "DIMENSIONS %ld %ld %ld\n", MY, MX, (long)1);
+  fprintf(f, "ORIGIN 0 0 0\n");
+  fprintf(f, "SPACING %le %le %le\n", p->dx, p->dy, in->dz);
+  fprintf(f, "POINT_DATA %ld\n", (long)MX * (long)MY);
+  for (int a = 0; a < p->nphases; a++) {
+    fprintf(f, "SCALARS %s double 1\n", in->phases[a]);
+    fprintf(f, "LOOKUP_TABLE default\n");
+    for (size_t i = 0; i < n; i++) {
+      double v = phi[a * n + i];
+      if (binary) { v = to_big_endian(v); fwrite(&v, 8, 1, f); }
+      else fprintf(f, "%le\n", v);
+    }
+    if (!binary) fprintf(f, "\n");
+  }
+  for (int k = 0; k < p->ncomp; k++) {
+    fprintf(f, "SCALARS Mu_%s double 1\n", in->components[k]);
+    fprintf(f, "LOOKUP_TABLE default\n");
+    for (size_t i = 0; i < n; i++) {
+      double v = mu[k * n + i];
+      if (binary) { v = to_big_endian(v); fwrite(&v, 8, 1, f); }
+      else fprintf(f, "%le\n", v);
+    }
+    if (!binary) fprintf(f, "\n");
+  }
+  if (in->writecomposition) {
+    for (int k = 0; k < p->ncomp; k++) {
+      fprintf(f, "SCALARS Composition_%s double 1\n", in->components[k]);
+      fprintf(f, "LOOKUP_TABLE default\n");
+      for (size_t i = 0; i < n; i++) {
+        double ph[P_MAX], m[C_MAX], comp = 0.0;
+        for (int a = 0; a < p->nphases; a++) ph[a] = phi[a * n + i];
+        for (int l = 0; l < p->ncomp; l++) m[l] = mu[l * n + i];
+        for (int b = 0; b < p->nphases; b++) comp += c_of_mu(p, m, p->T, b, k) * h_of_phi(ph, p->nphases, b);
+        if (binary) { double v = to_big_endian(comp); fwrite(&v, 8, 1, f); }
+        else fprintf(f, "%le \n", comp);
+      }
+      if (!binary) fprintf(f, "\n");
+    }
+  }
+  int bad = ferror(f);
+  fclose(f);
+  return bad ? GP_E_IO : GP_OK;
+}
+
+int gp_vtk_read(const char *path, const gp_input *in, double *phi, double *mu, int binary) {
+  const gp_params *p = &in->p;
+  FILE *f = fopen(path, binary ? "rb" : "r");
+  if (!f) return GP_E_IO;
+  size_t n = (size_t)p->mesh_x * p->mesh_y;
+  int ok = 1;
+  for (int i = 0; i < 8; i++) if (fscanf(f, "%*[^\n]\n") < 0) ok = 0;
+  for (int blk = 0; ok && blk < p->nphases + p->ncomp; blk++) {
+    double *dst = blk < p->nphases ? phi + (size_t)blk * n : mu + (size_t)(blk - p->nphases) * n;
+    if (fscanf(f, "%*[^\n]\n") < 0 || fscanf(f, "%*[^\n]\n") < 0) { ok = 0; break; }
+    for (size_t i = 0; i < n; i++) {
+      if (binary) {
+        double v;
+        if (fread(&v, 8, 1, f) != 1) { ok = 0; break; }
+        dst[i] = to_big_endian(v);
+      } else if (fscanf(f, "%le \n", &dst[i]) != 1) { ok = 0; break; }
+    }
+  }
+  fclose(f);
+  return ok ? GP_OK : GP_E_IO;
+}

@@ -1,0 +1,92 @@
+/*
+ * gpio.h -- host-side formats of the MicroSim serial grand-potential solver, kept
+ * so that the MI355X step is a drop-in behind the same files:
+ *
+ *   Input.in     key = value; lines (functions/reading_input_parameters.h:1-306,
+ *                functions/read_boundary_conditions.h:4-42)
+ *   Filling.in   FILLCUBE / FILLCYLINDER / FILLSPHERE / FILLELLIPSE lines
+ *                (functions/fill_domain.h:4-151, functions/filling.h:6-270)
+ *   VTK          legacy STRUCTURED_POINTS, ASCII (%le) or BINARY (big-endian
+ *                binary64) (solverloop/file_writer.h:23-416)
+ *
+ * plus the derivation of the constant tables the step reads
+ * (functions/functionF_01.h:169-230, functions/matrix.h:50-218,
+ *  functions/utility_functions.h:655-774).
+ */
+#ifndef GPIO_H_
+#define GPIO_H_
+
+#include "gpstep.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GP_NAME_LEN 51
+
+/* Everything Input.in sets that the step, the driver or the files use. */
+typedef struct gp_input {
+  gp_params p;                    /* grid, model switches, raw tables, BC tables        */
+  int    dimension;               /* DIMENSION                                          */
+  long   mesh_z;                  /* MESH_Z                                             */
+  double dz;                      /* DELTA_Z                                            */
+  long   ntimesteps, saveT, nsmooth, starttime, restart, track_progress;
+  int    ascii;                   /* WRITEFORMAT = ASCII -> 1, BINARY -> 0              */
+  int    writecomposition;        /* Writecomposition                                   */
+  int    shift, shiftj;           /* Shift, Shiftj                                      */
+  int    noise_phasefield;        /* Noise_phasefield                                   */
+  double amp_noise_phase;         /* Amp_Noise_Phase                                    */
+  int    function_f, function_w, function_anisotropy, fold;
+  int    tempgrady;               /* set when ISOTHERMAL = 0                            */
+  double tempgrad[5];             /* Tempgrady = {base, DeltaT, distance, offset, velocity} */
+  double Tfill, R, V, temperature_scale;
+  double bc_value[3][4];          /* BOUNDARY_VALUE (phi, mu, T) per face               */
+  int    bc_type_T[4];            /* BOUNDARY {T,...} types                             */
+  int    have_gamma, have_tau, have_dab, have_gamma_abc;
+  int    rot_given[GP_MAX_PHASES][GP_MAX_PHASES];
+  int    bc_proxy_set[2][4];      /* BOUNDARY_VALUE assigned proxies for field/face      */
+  int    ncomponents_names, nphases_names;
+  char   phases[GP_MAX_PHASES][GP_NAME_LEN];       /* PHASES (leading blanks kept)   */
+  char   components[GP_MAX_COMP + 1][GP_NAME_LEN]; /* COMPONENTS                     */
+  int    unknown_keys;            /* lines reported as unrecognised                     */
+} gp_input;
+
+/* Parse Input.in.  Returns GP_OK, GP_E_IO (file missing) or GP_E_ARG / GP_E_UNSUPPORTED
+   with a message in err.  Follows the reference's line grammar, key order
+   dependence and blank-line behaviour; unknown keys are counted, and printed to
+   stdout when verbose is set, as the reference does. */
+int gp_read_input(const char *path, gp_input *in, int verbose, char *err, size_t errlen);
+
+/* Fill the derived tables of p (cmu, Beq, dBbdT, B, Cf) exactly as
+   init_propertymatrices(T) leaves them.  Returns GP_OK or GP_E_ARG. */
+int gp_derive(gp_params *p);
+
+/* Check that p can be stepped by this build (isothermal, F_01, copy-type
+   boundaries with interior proxies, supported P/C).  GP_OK or a code with message. */
+int gp_check_supported(const gp_params *p, char *err, size_t errlen);
+
+/* Initial condition from Filling.in into interior arrays phi[P][MX][MY],
+   mu[C][MX][MY] (fill_domain + fill_composition_cube). */
+int gp_fill_domain(const gp_input *in, const char *filling_path,
+                   double *phi, double *mu, int verbose, char *err, size_t errlen);
+
+/* Legacy VTK of interior arrays.  binary = 1 writes big-endian binary64 blocks
+   (write_cells_vtk_2D_binary), 0 writes %le text (write_cells_vtk_2D).
+   The composition block (Writecomposition) is written when in->writecomposition. */
+int gp_vtk_write(const char *path, const gp_input *in, const double *phi,
+                 const double *mu, int binary);
+int gp_vtk_read(const char *path, const gp_input *in, double *phi, double *mu,
+                int binary);
+
+/* The reference's LU inverse with partial pivoting (matrix.h:50-114), including
+   its in-place rewrite of the input matrix; a and inv are n*n row-major, n <= 4. */
+void gp_matinv(double *a, double *inv, int n);
+
+/* Chemical potential Mu_a(c) at temperature T (functionF_01.h:19-29). */
+double gp_mu_of_c(const gp_params *p, const double *c, double T, int a, int i);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPIO_H_ */

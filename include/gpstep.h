@@ -1,0 +1,166 @@
+/*
+ * gpstep.h -- C ABI of the MI355X grand-potential 2-D finite-difference step.
+ *
+ * Drop-in for the time-loop body of the MicroSim serial grand-potential solver
+ * (Grand_potential_Finite_difference_2D_serial):
+ *
+ *     solverloop_phasefield(start,end);   apply_boundary_conditions(0);
+ *     solverloop_concentration(start,end); apply_boundary_conditions(0);
+ *
+ * (microsim_gp.c:103-106).  In the reference all data travels through globals
+ * (functions/global_vars.h); here it travels through a gp_params block of
+ * host-derived constants and an opaque context that owns the device fields.
+ *
+ * Conventions
+ *   - P = NUMPHASES (the last phase, P-1, is the liquid / reference phase),
+ *     C = NUMCOMPONENTS-1 independent diffusion potentials.
+ *   - Host-side field arrays are interior-only, planar, x-major with y fastest:
+ *       phi[a][x][y] = phi[(a*mesh_x + x)*mesh_y + y],  a < P
+ *       mu [k][x][y] = mu [(k*mesh_x + x)*mesh_y + y],  k < C
+ *     which is the order of the reference's VTK blocks (file_writer.h:86-111).
+ *   - Every entry point returns 0 on success or a negative GP_E_* code; no entry
+ *     point exits the process.  gp_strerror() names the code.
+ *   - Contexts share nothing; calls on different contexts may run concurrently
+ *     from different host threads.
+ */
+#ifndef GPSTEP_H_
+#define GPSTEP_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GP_MAX_PHASES 16   /* P limit of the parameter block               */
+#define GP_MAX_COMP   4    /* C limit (NUMCOMPONENTS-1) of the parameter block */
+
+/* error codes */
+#define GP_OK              0
+#define GP_E_ARG          -1   /* bad argument / inconsistent parameters         */
+#define GP_E_UNSUPPORTED  -2   /* valid input that this build does not implement */
+#define GP_E_HIP          -3   /* HIP runtime error                              */
+#define GP_E_NOMEM        -4   /* host or device allocation failed               */
+#define GP_E_NONFINITE    -5   /* non-finite value found in the fields           */
+#define GP_E_IO           -6   /* file could not be opened / parsed              */
+#define GP_E_NODEVICE     -7   /* no HIP device visible                          */
+
+/* chemical-potential update flavour (calculate_divergence_concentration.h:93-125) */
+#define GP_MU_GENERAL  0   /* LU inverse of dc/dmu (matrix.h:50-114)  */
+#define GP_MU_BINARY   1   /* scalar 1/dcdmu                          */
+#define GP_MU_TERNARY  2   /* closed-form 2x2 inverse                 */
+#define GP_MU_DILUTE   3   /* diagonal inverse                        */
+
+/* boundary types (utility_functions.h:471-653): 0 free, 1 Neumann, 2 Dirichlet, 3 periodic */
+#define GP_BC_FREE      0
+#define GP_BC_NEUMANN   1
+#define GP_BC_DIRICHLET 2
+#define GP_BC_PERIODIC  3
+
+/* faces, in the reference's boundary[] numbering */
+#define GP_FACE_XMIN 0
+#define GP_FACE_XMAX 1
+#define GP_FACE_YMAX 2
+#define GP_FACE_YMIN 3
+
+/* data types of a context */
+#define GP_F64 0   /* bulk stencil in binary64 (parity path)                     */
+#define GP_F32 1   /* bulk stencil in binary32, compensated mu, FP64 reductions  */
+
+/*
+ * Parameter block: everything one step reads.  The tables hold the values the
+ * reference keeps in globals after reading_input_parameters() and
+ * init_propertymatrices(T) (functions/functionF_01.h:169-230); gp_derive() in
+ * gpio.h fills the derived ones from the raw ones.  Unused entries must be 0.
+ */
+typedef struct gp_params {
+  long   mesh_x, mesh_y;          /* MESH_X, MESH_Y (interior cells)              */
+  double dx, dy, dt;              /* DELTA_X, DELTA_Y, DELTA_t                    */
+  int    nphases;                 /* P                                            */
+  int    ncomp;                   /* C = NUMCOMPONENTS-1                          */
+  int    obstacle;                /* 1: obstacle potential (Function_W=1), 0: well (Function_W=2) */
+  int    anisotropy;              /* Function_anisotropy: 0 isotropic, 1 four-fold cubic */
+  int    mu_solver;               /* GP_MU_*                                      */
+  int    isothermal;              /* ISOTHERMAL (this build steps isothermal runs) */
+  double epsilon;                 /* interface width parameter                    */
+  double tau;                     /* scalar relaxation constant (functionTau.h:13) */
+  double T, Teq;                  /* temperature, equilibrium temperature         */
+  /* ghost-layer copies: bc_type[field][face], field 0 = phi, 1 = mu; proxy[field][face][j]
+     is the padded index copied into ghost layer points[face][j] (assign_boundary_points_values) */
+  int    bc_type[2][4];
+  long   bc_proxy[2][4][3];
+  /* phase-pair tables */
+  double gamma[GP_MAX_PHASES][GP_MAX_PHASES];                     /* GAMMA        */
+  double gamma_abc[GP_MAX_PHASES][GP_MAX_PHASES][GP_MAX_PHASES];  /* Gamma_abc    */
+  double tau_ab[GP_MAX_PHASES][GP_MAX_PHASES];                    /* Tau          */
+  double dab[GP_MAX_PHASES][GP_MAX_PHASES];                       /* dab          */
+  double rot[GP_MAX_PHASES][GP_MAX_PHASES][3][3];                 /* Rotation_matrix (after inversion) */
+  double rot_inv[GP_MAX_PHASES][GP_MAX_PHASES][3][3];             /* Inv_Rotation_matrix */
+  /* thermodynamic tables */
+  double diffusivity[GP_MAX_PHASES][GP_MAX_COMP][GP_MAX_COMP];    /* DIFFUSIVITY  */
+  double A[GP_MAX_PHASES][GP_MAX_COMP][GP_MAX_COMP];              /* A            */
+  double ceq[GP_MAX_PHASES][GP_MAX_PHASES][GP_MAX_COMP];          /* ceq          */
+  double cfill[GP_MAX_PHASES][GP_MAX_PHASES][GP_MAX_COMP];        /* cfill        */
+  double slopes[GP_MAX_PHASES][GP_MAX_PHASES][GP_MAX_COMP];       /* slopes       */
+  /* derived by gp_derive() */
+  double cmu[GP_MAX_PHASES][GP_MAX_COMP][GP_MAX_COMP];            /* (dmu/dc)^-1  */
+  double Beq[GP_MAX_PHASES][GP_MAX_COMP];
+  double dBbdT[GP_MAX_PHASES][GP_MAX_COMP];
+  double B[GP_MAX_PHASES][GP_MAX_COMP];                           /* B(T)         */
+  double Cf[GP_MAX_PHASES];                                       /* C(T)         */
+} gp_params;
+
+/* Global diagnostics of the current state (replaces the TRACK_PROGRESS print of
+   microsim_gp.c:139-148, restricted to interior cells; sums in binary64). */
+typedef struct gp_stats {
+  double phi_max[GP_MAX_PHASES], phi_min[GP_MAX_PHASES];
+  double mu_max[GP_MAX_COMP],    mu_min[GP_MAX_COMP];
+  double phi_sum[GP_MAX_PHASES];            /* sum of phi_a over interior cells          */
+  double solute[GP_MAX_COMP];               /* sum_cells w * sum_a h_a(phi) c_a(mu)       */
+  double nonfinite;                         /* number of non-finite interior values      */
+} gp_stats;
+
+typedef struct gp_ctx gp_ctx;
+
+/* Create a context: validates p, copies it to the device, allocates the fields.
+   dtype is GP_F64 or GP_F32.  Replaces serialinfo_xy() + the allocation part of
+   main() (microsim_gp.c:43-47, solverloop/serialinfo_xy.h:1-66). */
+int  gp_create(const gp_params *p, int dtype, gp_ctx **out);
+void gp_destroy(gp_ctx *ctx);
+
+/* Load / read back the interior fields (host arrays in the layout above, binary64).
+   gp_set_state also fills the ghost layers, like apply_boundary_conditions(0)
+   after fill_domain / restart reading (microsim_gp.c:48-57,84). */
+int  gp_set_state(gp_ctx *ctx, const double *phi, const double *mu);
+int  gp_get_state(gp_ctx *ctx, double *phi, double *mu);
+
+/* Same, with device pointers of the context's dtype (float or double) holding the
+   interior layout above; ordered on the context's stream. */
+int  gp_set_state_device(gp_ctx *ctx, const void *phi, const void *mu);
+int  gp_get_state_device(gp_ctx *ctx, void *phi, void *mu);
+
+/* Advance nsteps explicit Euler steps: each step is the phase sweep, ghost copy,
+   concentration sweep and ghost copy of microsim_gp.c:103-106. */
+int  gp_step(gp_ctx *ctx, long nsteps);
+
+/* Diagnostics of the current state. */
+int  gp_stats_get(gp_ctx *ctx, gp_stats *st);
+
+/* Stream the context enqueues on (a hipStream_t; NULL = the default stream). */
+int  gp_set_stream(gp_ctx *ctx, void *hip_stream);
+
+/* Device-side time of the last gp_step call in milliseconds (hipEvents on the
+   context's stream, bracketing every kernel of that call). */
+int  gp_last_step_ms(gp_ctx *ctx, double *ms);
+
+/* Kernel path of a context: 0 = auto (fused LDS kernel), 1 = two-pass reference
+   kernels (phase kernel + concentration kernel). */
+int  gp_set_path(gp_ctx *ctx, int path);
+
+const char *gp_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPSTEP_H_ */

@@ -1,0 +1,749 @@
+/*
+ * gp_oracle.c -- TEST INFRASTRUCTURE ONLY (see gp_oracle.h).
+ *
+ * Clean-room CPU restatement, in binary64 and in the reference's operation
+ * order, of one explicit step of Grand_potential_Finite_difference_2D_serial:
+ *
+ *   stage 1  face/centre gradients of phi     solverloop/calculate_gradients.h:92-131
+ *   stage 2  delta-phi per cell               solverloop/calculate_divergence_phasefield.h:4-62
+ *            with  dA/dphi, div dA/dgrad phi   functions/function_A_00.h:4-58,
+ *                                             functions/function_A_01.h:4-71,101-219,
+ *                                             functions/functionQ.h:4-69, functions/anisotropy_01.h:9-48
+ *            dw/dphi                          functions/functionW_01.h:6-39, functionW_02.h:4-33
+ *            dpsi, c(mu), Mu(c)               functions/functionF_01.h:5-57
+ *            h, dh/dphi                       functions/functionH.h:1-36
+ *            tau(phi)                         functions/functionTau.h:1-16
+ *            simplex projection               solverloop/simplex_projection.h:1-49
+ *   stage 3  mu gradients, face mobility       solverloop/calculate_gradients.h:132-185, functions/functionD.h:1-15
+ *   stage 4  antitrapping fluxes              solverloop/calculate_fluxes_concentration.h:4-145
+ *   stage 5  mu and phi update                solverloop/calculate_divergence_concentration.h:4-153
+ *   stage 6  ghost copies                     solverloop/boundary_serial.h:8-203
+ *
+ * The reference sweeps x-layers through a four-slot ring buffer and updates the
+ * grid in place.  Every quantity it reads during a step is a value of the
+ * pre-step fields (the update of layer x-1 happens after everything that reads
+ * layer x-1 has been computed), so whole-grid stages give the same numbers.
+ * Values that the ring buffer leaves stale (gradient slots of layers 0-2 during
+ * the concentration sweep, the unwritten gidy = rows_y-1 slot) reach only ghost
+ * cells, which the ghost copies overwrite for Neumann and periodic faces; the
+ * oracle therefore computes delta-phi on x,y in [2, rows-3] and updates only the
+ * interior.
+ */
+#include "gp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define PM GP_MAX_PHASES
+#define CM GP_MAX_COMP
+
+typedef struct grid {
+  const gp_params *p;
+  long rx, ry;
+  size_t n;
+  int P, C;
+} grid;
+
+#define AT(g, arr, a, x, y) (arr)[((size_t)(a) * (g)->rx + (x)) * (g)->ry + (y)]
+
+/* ---------------- thermodynamics (functionF_01.h, functionH.h) ------------ */
+
+static double f_c_mu(const gp_params *p, const double *mu, int a, int i) {
+  double s = 0.0;
+  for (int k = 0; k < p->ncomp; k++)
+    s += p->cmu[a][i][k] * (mu[k] - (p->Beq[a][k] + p->dBbdT[a][k] * (p->T - p->Teq)));
+  return s;
+}
+
+static double f_Mu(const gp_params *p, const double *c, int a, int i) {
+  double s = 2.0 * p->A[a][i][i] * c[i] + (p->Beq[a][i] + p->dBbdT[a][i] * (p->T - p->Teq));
+  for (int j = 0; j < p->ncomp; j++)
+    if (i != j) s += p->A[a][i][j] * c[j];
+  return s;
+}
+
+static double f_free_energy(const gp_params *p, const double *c, int a) {
+  double s = 0.0;
+  for (int i = 0; i < p->ncomp; i++) {
+    for (int j = 0; j < p->ncomp; j++)
+      if (i <= j) s += p->A[a][i][j] * c[i] * c[j];
+    s += p->B[a][i] * c[i];
+  }
+  s += p->Cf[a];
+  return s;
+}
+
+/* dh_b / dphi_a */
+static double f_dhphi(const double *phi, int P, int b, int a) {
+  double s = 0.0;
+  if (a == b) {
+    s = 6.0 * phi[a] * (1.0 - phi[a]);
+    for (int e = 0; e < P; e++)
+      for (int f = 0; f < P; f++)
+        if (e != b && f != b && e < f) s += 2.0 * phi[e] * phi[f];
+  } else {
+    for (int e = 0; e < P; e++)
+      if (e != b && e != a) s += 2.0 * phi[e];
+    s *= phi[b];
+  }
+  return s;
+}
+
+static double f_hphi(const double *phi, int P, int a) {
+  double s1 = 0.0;
+  double s = 3.0 * phi[a] * phi[a] - 2.0 * phi[a] * phi[a] * phi[a];
+  for (int b = 0; b < P; b++)
+    for (int c = 0; c < P; c++)
+      if (b != a && c != a && b < c) s1 += phi[b] * phi[c];
+  s1 *= 2.0 * phi[a];
+  return s + s1;
+}
+
+static double f_dpsi(const gp_params *p, const double *mu, const double *phi, int a) {
+  double sum = 0.0;
+  for (int b = 0; b < p->nphases; b++) {
+    double c[CM], psi = 0.0;
+    for (int k = 0; k < p->ncomp; k++) {
+      c[k] = f_c_mu(p, mu, b, k);
+      psi -= mu[k] * c[k];
+    }
+    psi += f_free_energy(p, c, b);
+    sum += psi * f_dhphi(phi, p->nphases, b, a);
+  }
+  return sum;
+}
+
+static double f_tau(const gp_params *p, const double *phi) {
+  double s = 0.0, s1 = 0.0;
+  for (int a = 0; a < p->nphases; a++)
+    for (int b = 0; b < p->nphases; b++)
+      if (a < b) {
+        s += p->tau_ab[a][b] * phi[a] * phi[b];
+        s1 += phi[a] * phi[b];
+      }
+  if (s1) return s / s1;
+  return p->tau;
+}
+
+/* ---------------- anisotropy (functionQ.h, anisotropy_01.h) -------------- */
+
+static void rot2(const double R[3][3], const double *q, double *out) {
+  for (int i = 0; i < 2; i++) {
+    double s = 0;
+    for (int j = 0; j < 2; j++) s = s + R[i][j] * q[j];
+    out[i] = s;
+  }
+}
+
+static void f_dadq(const gp_params *p, const double *q, double *d, int a, int b) {
+  double qz = 0.0;
+  double qx3 = q[0] * q[0] * q[0], qy3 = q[1] * q[1] * q[1], qz3 = qz * qz * qz;
+  double q2 = q[0] * q[0] + q[1] * q[1] + qz * qz;
+  double q22 = q2 * q2, q23 = q2 * q2 * q2;
+  double q4 = qx3 * q[0] + qy3 * q[1] + qz3 * qz;
+  if (fabs(q2) > 1.0e-15) {
+    d[0] = 16.0 * p->dab[a][b] * (qx3 / q22 - q[0] * q4 / q23);
+    d[1] = 16.0 * p->dab[a][b] * (qy3 / q22 - q[1] * q4 / q23);
+  } else {
+    d[0] = 0.0;
+    d[1] = 0.0;
+  }
+}
+
+static double f_ac(const gp_params *p, const double *q, int a, int b) {
+  double qx2 = q[0] * q[0], qx4 = qx2 * qx2, qy2 = q[1] * q[1], qy4 = qy2 * qy2;
+  double qz2 = 0.0 * 0.0, qz4 = qz2 * qz2;
+  double q2 = qx2 + qy2 + qz2;
+  if (fabs(q2) > 1.0e-15) return 1.0 - p->dab[a][b] * (3.0 - 4.0 * (qx4 + qy4 + qz4) / (q2 * q2));
+  return 1.0;
+}
+
+/* ---------------- the step -------------------------------------------------- */
+
+typedef struct work {
+  double *gX, *sX, *gY, *sY, *cX, *cY; /* [P][rx][ry] */
+  double *dphi;                        /* [P][rx][ry] */
+  double *jX, *jY;                     /* [C][rx][ry] */
+  double *gmX, *gmY;                   /* [C][rx][ry] */
+  double *DmX, *DmY;                   /* [C*C][rx][ry] */
+} work;
+
+static void load_cell(const grid *g, const double *phi, const double *mu, long x, long y, double *ph, double *m) {
+  for (int a = 0; a < g->P; a++) ph[a] = AT(g, phi, a, x, y);
+  for (int k = 0; k < g->C; k++) m[k] = AT(g, mu, k, x, y);
+}
+
+/* stage 1: calculate_gradients_phasefield_2D over the whole padded grid */
+static void stage_gradients(const grid *g, const double *phi, work *w) {
+  const gp_params *p = g->p;
+  long rx = g->rx, ry = g->ry;
+#pragma omp parallel for schedule(static)
+  for (long x = 0; x < rx; x++)
+    for (int a = 0; a < g->P; a++)
+      for (long y = 0; y < ry; y++) {
+        double c = AT(g, phi, a, x, y);
+        if (x + 1 < rx) {
+          double f = AT(g, phi, a, x + 1, y);
+          AT(g, w->gX, a, x, y) = (f - c) / p->dx;
+          AT(g, w->sX, a, x, y) = 0.5 * (f + c);
+        }
+        if (y + 1 < ry) {
+          double r = AT(g, phi, a, x, y + 1);
+          AT(g, w->gY, a, x, y) = (r - c) / p->dy;
+          AT(g, w->sY, a, x, y) = 0.5 * (r + c);
+        }
+        if (y > 0 && y + 1 < ry)
+          AT(g, w->cY, a, x, y) = (AT(g, phi, a, x, y + 1) - AT(g, phi, a, x, y - 1)) / (2.0 * p->dy);
+      }
+#pragma omp parallel for schedule(static)
+  for (long x = 1; x < rx - 1; x++)
+    for (int a = 0; a < g->P; a++)
+      for (long y = 0; y < ry; y++)
+        AT(g, w->cX, a, x, y) = 0.5 * (AT(g, w->gX, a, x, y) + AT(g, w->gX, a, x - 1, y));
+}
+
+/* q on the x-face between (x,y) and (x+1,y) for the ordered pair (a,b), rotated */
+static void q_xface(const grid *g, const work *w, long x, long y, int a, int b, double *q) {
+  double u[2];
+  u[0] = (AT(g, w->sX, a, x, y) * AT(g, w->gX, b, x, y) - AT(g, w->sX, b, x, y) * AT(g, w->gX, a, x, y));
+  u[1] = 0.5 * (AT(g, w->sX, a, x, y) * (AT(g, w->cY, b, x, y) + AT(g, w->cY, b, x + 1, y)) -
+                AT(g, w->sX, b, x, y) * (AT(g, w->cY, a, x, y) + AT(g, w->cY, a, x + 1, y)));
+  rot2(g->p->rot[a][b], u, q);
+}
+
+/* q on the y-face between (x,y) and (x,y+1) */
+static void q_yface(const grid *g, const work *w, long x, long y, int a, int b, double *q) {
+  double u[2];
+  u[0] = 0.5 * (AT(g, w->sY, a, x, y) * (AT(g, w->cX, b, x, y) + AT(g, w->cX, b, x, y + 1)) -
+                AT(g, w->sY, b, x, y) * (AT(g, w->cX, a, x, y) + AT(g, w->cX, a, x, y + 1)));
+  u[1] = (AT(g, w->sY, a, x, y) * AT(g, w->gY, b, x, y) - AT(g, w->sY, b, x, y) * AT(g, w->gY, a, x, y));
+  rot2(g->p->rot[a][b], u, q);
+}
+
+static double dAdphi(const grid *g, const work *w, const double *ph, long x, long y, int a) {
+  const gp_params *p = g->p;
+  if (!p->anisotropy) {
+    double sum = 0.0;
+    for (int b = 0; b < g->P; b++) {
+      if (b == a) continue;
+      double sp = 0.0;
+      sp += (ph[a] * AT(g, w->cX, b, x, y) - ph[b] * AT(g, w->cX, a, x, y)) * AT(g, w->cX, b, x, y);
+      sp += (ph[a] * AT(g, w->cY, b, x, y) - ph[b] * AT(g, w->cY, a, x, y)) * AT(g, w->cY, b, x, y);
+      sp *= p->gamma[a][b];
+      sum += sp;
+    }
+    return 2.0 * sum;
+  }
+  double sum1 = 0.0, sum2 = 0.0;
+  for (int b = 0; b < g->P; b++) {
+    if (b == a) continue;
+    double u[2], q[2], d[2], dq[2], rdq[2];
+    u[0] = (ph[a] * AT(g, w->cX, b, x, y) - ph[b] * AT(g, w->cX, a, x, y));
+    u[1] = (ph[a] * AT(g, w->cY, b, x, y) - ph[b] * AT(g, w->cY, a, x, y));
+    rot2(p->rot[a][b], u, q);
+    f_dadq(p, q, d, a, b);
+    double qab2 = (q[0] * q[0] + q[1] * q[1] + 0.0 * 0.0);
+    double ac1 = f_ac(p, q, a, b), ac2 = ac1 * ac1;
+    dq[0] = AT(g, w->cX, b, x, y);
+    dq[1] = AT(g, w->cY, b, x, y);
+    rot2(p->rot[a][b], dq, rdq);
+    double s1 = 0.0, s2 = 0.0;
+    for (int dir = 0; dir < 2; dir++) s1 += q[dir] * rdq[dir];
+    for (int dir = 0; dir < 2; dir++) s2 += d[dir] * rdq[dir];
+    s1 *= p->gamma[a][b] * ac2;
+    s2 *= p->gamma[a][b] * ac1 * qab2;
+    sum1 += s1;
+    sum2 += s2;
+  }
+  return 2.0 * (sum1 + sum2);
+}
+
+/* (sa*gb - sb*ga) on a face, unrotated */
+#define FLUXX(a, b, x, y) (AT(g, w->sX, a, x, y) * AT(g, w->gX, b, x, y) - AT(g, w->sX, b, x, y) * AT(g, w->gX, a, x, y))
+#define FLUXY(a, b, x, y) (AT(g, w->sY, a, x, y) * AT(g, w->gY, b, x, y) - AT(g, w->sY, b, x, y) * AT(g, w->gY, a, x, y))
+
+static double divdAdgradphi(const grid *g, const work *w, long x, long y, int a) {
+  const gp_params *p = g->p;
+  double dx = p->dx, dy = p->dy;
+  if (!p->anisotropy) {
+    double sum = 0.0;
+    for (int b = 0; b < g->P; b++) {
+      if (b == a) continue;
+      double dv = (FLUXX(a, b, x, y) * (-AT(g, w->sX, b, x, y)) - (FLUXX(a, b, x - 1, y) * (-AT(g, w->sX, b, x - 1, y)))) / dx;
+      dv += (FLUXY(a, b, x, y) * (-AT(g, w->sY, b, x, y)) - (FLUXY(a, b, x, y - 1) * (-AT(g, w->sY, b, x, y - 1)))) / dy;
+      dv *= p->gamma[a][b];
+      sum += dv;
+    }
+    return 2.0 * sum;
+  }
+  double sum1 = 0.0, sum2 = 0.0, div3 = 0.0;
+  for (int b = 0; b < g->P; b++) {
+    if (b == a) continue;
+    double qf[2], qb[2], qr[2], ql[2];
+    q_xface(g, w, x, y, a, b, qf);
+    q_xface(g, w, x - 1, y, a, b, qb);
+    q_yface(g, w, x, y, a, b, qr);
+    q_yface(g, w, x, y - 1, a, b, ql);
+    double acf = f_ac(p, qf, a, b), acb = f_ac(p, qb, a, b), acr = f_ac(p, qr, a, b), acl = f_ac(p, ql, a, b);
+    double acf2 = acf * acf, acb2 = acb * acb, acr2 = acr * acr, acl2 = acl * acl;
+    double d1 = ((acf2 * FLUXX(a, b, x, y) * (-AT(g, w->sX, b, x, y))) - (acb2 * FLUXX(a, b, x - 1, y) * (-AT(g, w->sX, b, x - 1, y)))) / dx;
+    d1 += ((acr2 * FLUXY(a, b, x, y) * (-AT(g, w->sY, b, x, y))) - (acl2 * FLUXY(a, b, x, y - 1) * (-AT(g, w->sY, b, x, y - 1)))) / dy;
+    d1 *= p->gamma[a][b];
+    sum1 += d1;
+    double df[2], db[2], dr[2], dl[2], rf[2], rb[2], rr[2], rl[2];
+    f_dadq(p, qf, df, a, b); rot2(p->rot_inv[a][b], df, rf);
+    f_dadq(p, qb, db, a, b); rot2(p->rot_inv[a][b], db, rb);
+    f_dadq(p, qr, dr, a, b); rot2(p->rot_inv[a][b], dr, rr);
+    f_dadq(p, ql, dl, a, b); rot2(p->rot_inv[a][b], dl, rl);
+    double qf2 = qf[0] * qf[0] + qf[1] * qf[1] + 0.0 * 0.0;
+    double qb2 = qb[0] * qb[0] + qb[1] * qb[1] + 0.0 * 0.0;
+    double qr2 = qr[0] * qr[0] + qr[1] * qr[1] + 0.0 * 0.0;
+    double ql2 = ql[0] * ql[0] + ql[1] * ql[1] + 0.0 * 0.0;
+    double d2 = (acf * qf2 * rf[0] * (-AT(g, w->sX, b, x, y)) - acb * qb2 * rb[0] * (-AT(g, w->sX, b, x - 1, y))) / dx;
+    d2 += (acr * qr2 * rr[1] * (-AT(g, w->sY, b, x, y)) - acl * ql2 * rl[1] * (-AT(g, w->sY, b, x, y - 1))) / dy;
+    d2 *= p->gamma[a][b];
+    sum2 += d2;
+  }
+  return (2.0 * (sum1 + sum2) + div3);
+}
+
+static double dwdphi(const gp_params *p, const double *ph, const double *divphi, int a) {
+  int P = p->nphases;
+  double sum = 0.0;
+  if (p->obstacle) {
+    for (int b = 0; b < P; b++)
+      if (b != a && fabs(divphi[b]) > 0.0) {
+        if ((ph[a] * ph[b]) >= 0.00000) sum += p->gamma[a][b] * (ph[b]);
+        else sum += -p->gamma[a][b] * (ph[b]);
+      }
+    sum *= 16.0 / (M_PI * M_PI);
+  } else {
+    for (int b = 0; b < P; b++)
+      if (b != a && fabs(divphi[b]) > 0.0) sum += 2.0 * p->gamma[a][b] * (ph[b] * ph[b] * ph[a]);
+    sum *= 9.0;
+  }
+  for (int b = 0; b < P; b++)
+    for (int c = 0; c < P; c++)
+      if (b != a && c != a && b < c && fabs(divphi[b]) > 0.0 && fabs(divphi[c]) > 0.0) {
+        double pbc = ph[b] * ph[c];
+        if (ph[a] * pbc >= 0.00000) sum += p->gamma_abc[a][b][c] * pbc;
+        else sum += -p->gamma_abc[a][b][c] * pbc;
+      }
+  return sum;
+}
+
+static void projection(int P, const double *ph, const double *divphi, double *dp) {
+  double Deltaphi = 0.0, Da = 0.0, sum_phib = 0.0;
+  long count = 0;
+  for (int a = 0; a < P; a++)
+    if ((fabs(divphi[a]) > 0.0) && (ph[a] + dp[a]) > 0.0 && (ph[a] + dp[a]) < 1.0) {
+      count++;
+      sum_phib += ph[a];
+    }
+  for (int a = 0; a < P; a++)
+    if ((ph[a] + dp[a]) < 0.0) {
+      Da = fabs(dp[a] + ph[a]);
+      dp[a] += Da;
+      Deltaphi += Da;
+    }
+  if (Deltaphi > 0.0) {
+    for (int b = 0; b < P; b++)
+      if ((fabs(divphi[b]) > 0.0) && ((ph[b] + dp[b]) > 0.0) && ((ph[b] + dp[b]) < 1.0)) {
+        if (fabs(sum_phib) > 0.0) dp[b] -= Deltaphi * (ph[b]) / (sum_phib);
+        else dp[b] -= Deltaphi / (count);
+      }
+  }
+  for (int a = 0; a < P; a++)
+    if (fabs(divphi[a]) > 0.0 && (ph[a] + dp[a]) > 1.0) {
+      dp[a] = (1.0 - ph[a]);
+      for (int b = 0; b < P; b++)
+        if ((fabs(divphi[b]) > 0.0) && b != a) dp[b] = -ph[b];
+      break;
+    }
+}
+
+/* stage 2 */
+static void stage_dphi(const grid *g, const double *phi, const double *mu, work *w) {
+  const gp_params *p = g->p;
+  int P = g->P;
+  memset(w->dphi, 0, sizeof(double) * g->n * P);
+#pragma omp parallel for schedule(static)
+  for (long x = 2; x <= g->rx - 3; x++)
+    for (long y = 2; y <= g->ry - 3; y++) {
+      double ph[PM], m[CM], divphi[PM], lam[PM], dp[PM];
+      load_cell(g, phi, mu, x, y, ph, m);
+      for (int a = 0; a < P; a++) {
+        divphi[a] = (AT(g, w->gX, a, x, y) - AT(g, w->gX, a, x - 1, y)) / p->dx;
+        divphi[a] += (AT(g, w->gY, a, x, y) - AT(g, w->gY, a, x, y - 1)) / p->dy;
+      }
+      double sl = 0.0;
+      long active = 0;
+      for (int a = 0; a < P; a++)
+        if (fabs(divphi[a]) > 0.0) {
+          lam[a] = p->epsilon * (-dAdphi(g, w, ph, x, y, a) + divdAdgradphi(g, w, x, y, a)) -
+                   (1.0 / p->epsilon) * dwdphi(p, ph, divphi, a) - f_dpsi(p, m, ph, a);
+          sl += lam[a];
+          active++;
+        }
+      if (active) sl /= active;
+      for (int a = 0; a < P; a++) {
+        if (fabs(divphi[a]) > 0.0) dp[a] = p->dt * (lam[a] - sl) / (f_tau(p, ph) * p->epsilon);
+        else dp[a] = 0.0;
+      }
+      projection(P, ph, divphi, dp);
+      for (int a = 0; a < P; a++) AT(g, w->dphi, a, x, y) = dp[a];
+    }
+}
+
+static double s_phi(const gp_params *p, const double *ph, int a) {
+  if (ph[a] * (1.0 - ph[a]) > 0.0)
+    return ph[a] * (1.0 - f_hphi(ph, p->nphases, a)) / (sqrt(ph[a] * (1.0 - ph[a])));
+  return 0.0;
+}
+
+/* stage 3: mu gradients and face mobility (calculate_gradients_concentration_2D, D()) */
+static void cell_D(const gp_params *p, const double *ph, double D[CM][CM]) {
+  for (int i = 0; i < p->ncomp; i++)
+    for (int j = 0; j < p->ncomp; j++) {
+      double s = 0.0;
+      for (int b = 0; b < p->nphases; b++)
+        for (int k = 0; k < p->ncomp; k++) s += p->diffusivity[b][i][k] * p->cmu[b][k][j] * ph[b];
+      D[i][j] = s;
+    }
+}
+
+static void stage_mu_faces(const grid *g, const double *phi, const double *mu, work *w) {
+  const gp_params *p = g->p;
+  int C = g->C;
+#pragma omp parallel for schedule(static)
+  for (long x = 2; x <= g->rx - 4; x++)
+    for (long y = 2; y <= g->ry - 4; y++) {
+      double ph[PM], m[CM], phf[PM], mf[CM], phr[PM], mr[CM];
+      double D[CM][CM], Df[CM][CM], Dr[CM][CM];
+      load_cell(g, phi, mu, x, y, ph, m);
+      load_cell(g, phi, mu, x + 1, y, phf, mf);
+      load_cell(g, phi, mu, x, y + 1, phr, mr);
+      cell_D(p, ph, D);
+      cell_D(p, phf, Df);
+      cell_D(p, phr, Dr);
+      for (int k = 0; k < C; k++) {
+        AT(g, w->gmX, k, x, y) = (mf[k] - m[k]) / p->dx;
+        AT(g, w->gmY, k, x, y) = (mr[k] - m[k]) / p->dy;
+        for (int l = 0; l < C; l++) {
+          AT(g, w->DmX, k * C + l, x, y) = (Df[k][l] + D[k][l]) * 0.5;
+          AT(g, w->DmY, k * C + l, x, y) = (Dr[k][l] + D[k][l]) * 0.5;
+        }
+      }
+    }
+}
+
+/* stage 4: antitrapping fluxes jat[X] on (x,y)|(x+1,y), jat[Y] on (x,y)|(x,y+1) */
+static void stage_fluxes(const grid *g, const double *phi, const double *mu, work *w) {
+  const gp_params *p = g->p;
+  int P = g->P, C = g->C, L = P - 1;
+#pragma omp parallel for schedule(static)
+  for (long x = 2; x <= g->rx - 4; x++)
+    for (long y = 2; y <= g->ry - 4; y++) {
+      double ph[PM], m[CM], phf[PM], mf[CM], phr[PM], mr[CM];
+      load_cell(g, phi, mu, x, y, ph, m);
+      load_cell(g, phi, mu, x + 1, y, phf, mf);
+      load_cell(g, phi, mu, x, y + 1, phr, mr);
+      double jX[CM], jY[CM];
+      double gxl = 0.5 * (AT(g, w->cX, L, x, y) + AT(g, w->cX, L, x, y + 1));
+      double ngyl = sqrt(gxl * gxl + AT(g, w->gY, L, x, y) * AT(g, w->gY, L, x, y));
+      double gyl = 0.5 * (AT(g, w->cY, L, x, y) + AT(g, w->cY, L, x + 1, y));
+      double ngxl = sqrt(gyl * gyl + AT(g, w->gX, L, x, y) * AT(g, w->gX, L, x, y));
+      for (int k = 0; k < C; k++) jX[k] = jY[k] = 0.0;
+      for (int a = 0; a < L; a++) {
+        double gx = 0.5 * (AT(g, w->cX, a, x, y) + AT(g, w->cX, a, x, y + 1));
+        double ngp = sqrt(gx * gx + AT(g, w->gY, a, x, y) * AT(g, w->gY, a, x, y));
+        double s_r = 0, s_c = 0, s_f = 0;
+        if (p->obstacle) {
+          s_r = s_phi(p, phr, a);
+          s_c = s_phi(p, ph, a);
+          s_f = s_phi(p, phf, a);
+        }
+        for (int k = 0; k < C; k++) {
+          if (ngp != 0.0 && y != 0 && y != (p->mesh_y - 2)) {
+            double clr = f_c_mu(p, mr, L, k), clc = f_c_mu(p, m, L, k);
+            double csr = f_c_mu(p, mr, a, k), csc = f_c_mu(p, m, a, k);
+            double sp = AT(g, w->gY, a, x, y) * AT(g, w->gY, L, x, y);
+            sp += gx * gxl;
+            if (ngyl > 0.0) sp /= (ngyl * ngp);
+            double coef = (1.0 - p->diffusivity[a][k][k] / p->diffusivity[L][k][k]);
+            if (!p->obstacle)
+              jY[k] += coef * fabs(sp) * (0.5 / sqrt(2.0)) * ((clr - csr) * AT(g, w->dphi, a, x, y + 1) + (clc - csc) * AT(g, w->dphi, a, x, y)) * AT(g, w->gY, a, x, y) / ngp;
+            else
+              jY[k] += coef * fabs(sp) * 0.5 * (s_r * (clr - csr) * AT(g, w->dphi, a, x, y + 1) + s_c * (clc - csc) * AT(g, w->dphi, a, x, y)) * AT(g, w->gY, a, x, y) / ngp;
+          }
+        }
+        double gy = 0.5 * (AT(g, w->cY, a, x, y) + AT(g, w->cY, a, x + 1, y));
+        ngp = sqrt(gy * gy + AT(g, w->gX, a, x, y) * AT(g, w->gX, a, x, y));
+        for (int k = 0; k < C; k++) {
+          if (ngp != 0.0) {
+            double clf = f_c_mu(p, mf, L, k), clc = f_c_mu(p, m, L, k);
+            double csf = f_c_mu(p, mf, a, k), csc = f_c_mu(p, m, a, k);
+            double sp = AT(g, w->gX, a, x, y) * AT(g, w->gX, L, x, y);
+            sp += gy * gyl;
+            if (ngxl > 0.0) sp /= (ngxl * ngp);
+            double coef = (1.0 - p->diffusivity[a][k][k] / p->diffusivity[L][k][k]);
+            if (!p->obstacle)
+              jX[k] += coef * fabs(sp) * (0.5 / sqrt(2.0)) * ((clf - csf) * AT(g, w->dphi, a, x + 1, y) + (clc - csc) * AT(g, w->dphi, a, x, y)) * (AT(g, w->gX, a, x, y)) / ngp;
+            else
+              jX[k] += coef * fabs(sp) * 0.5 * (s_f * (clf - csf) * AT(g, w->dphi, a, x + 1, y) + s_c * (clc - csc) * AT(g, w->dphi, a, x, y)) * (AT(g, w->gX, a, x, y)) / ngp;
+          }
+        }
+      }
+      for (int k = 0; k < C; k++) {
+        AT(g, w->jX, k, x, y) = jX[k];
+        AT(g, w->jY, k, x, y) = jY[k];
+      }
+    }
+}
+
+/* stage 5: update of the interior, into separate output arrays */
+static void stage_update(const grid *g, const double *phi, const double *mu, work *w, double *phi_o, double *mu_o) {
+  const gp_params *p = g->p;
+  int P = g->P, C = g->C;
+  long MX = p->mesh_x, MY = p->mesh_y;
+  double jscale = p->obstacle ? (-0.25 * M_PI * p->epsilon) : (-1.0 * p->epsilon);
+#pragma omp parallel for schedule(static)
+  for (long x = 3; x <= MX + 2; x++)
+    for (long y = 3; y <= MY + 2; y++) {
+      double ph[PM], m[CM], dp[PM], sum[CM], divflux[CM], divjat[CM], dmu[CM];
+      double dcdmu[CM][CM];
+      load_cell(g, phi, mu, x, y, ph, m);
+      for (int a = 0; a < P; a++) dp[a] = AT(g, w->dphi, a, x, y);
+      for (int k = 0; k < C; k++) {
+        sum[k] = 0.0;
+        for (int l = 0; l < C; l++) dcdmu[k][l] = 0.0;
+      }
+      for (int a = 0; a < P; a++) {
+        double sdh = 0.0;
+        for (int b = 0; b < P; b++) sdh += f_dhphi(ph, P, a, b) * dp[b];
+        for (int k = 0; k < C; k++) sum[k] += f_c_mu(p, m, a, k) * sdh;
+      }
+      for (int k = 0; k < C; k++) {
+        divflux[k] = 0.0;
+        for (int l = 0; l < C; l++) {
+          divflux[k] += (AT(g, w->DmX, k * C + l, x, y) * AT(g, w->gmX, l, x, y) - AT(g, w->DmX, k * C + l, x - 1, y) * AT(g, w->gmX, l, x - 1, y)) / p->dx;
+          divflux[k] += (AT(g, w->DmY, k * C + l, x, y) * AT(g, w->gmY, l, x, y) - AT(g, w->DmY, k * C + l, x, y - 1) * AT(g, w->gmY, l, x, y - 1)) / p->dy;
+        }
+        divjat[k] = (AT(g, w->jX, k, x, y) - AT(g, w->jX, k, x - 1, y)) * jscale / p->dx;
+        divjat[k] += (AT(g, w->jY, k, x, y) - AT(g, w->jY, k, x, y - 1)) * jscale / p->dy;
+      }
+      int bulk = -1;
+      for (int a = 0; a < P; a++)
+        if (ph[a] == 1.0) { bulk = a; break; }
+      if (bulk >= 0) {
+        double cn[CM];
+        for (int k = 0; k < C; k++) cn[k] = f_c_mu(p, m, bulk, k) + p->dt * (divflux[k]) - divjat[k];
+        for (int k = 0; k < C; k++) AT(g, mu_o, k, x, y) = f_Mu(p, cn, bulk, k);
+      } else {
+        double dc[CM];
+        for (int k = 0; k < C; k++) {
+          dc[k] = p->dt * (divflux[k]) - divjat[k] - sum[k];
+          for (int l = 0; l < C; l++)
+            for (int a = 0; a < P; a++) dcdmu[k][l] += p->cmu[a][k][l] * f_hphi(ph, P, a);
+        }
+        double mo[CM];
+        for (int k = 0; k < C; k++) mo[k] = m[k];
+        if (p->mu_solver == GP_MU_BINARY) {
+          double inv = 1.0 / dcdmu[0][0];
+          dmu[0] = dc[0] * inv;
+          mo[0] += dmu[0];
+        } else if (p->mu_solver == GP_MU_TERNARY) {
+          double DET = dcdmu[0][0] * dcdmu[1][1] - dcdmu[0][1] * dcdmu[1][0];
+          double inv[2][2];
+          inv[0][0] = dcdmu[1][1] / DET;
+          inv[1][1] = dcdmu[0][0] / DET;
+          inv[0][1] = -dcdmu[0][1] / DET;
+          inv[1][0] = -dcdmu[1][0] / DET;
+          for (int k = 0; k < C; k++) {
+            dmu[k] = 0.0;
+            for (int l = 0; l < C; l++) dmu[k] += inv[k][l] * dc[l];
+          }
+          for (int k = 0; k < C; k++) mo[k] = dmu[k] + m[k];
+        } else if (p->mu_solver == GP_MU_DILUTE) {
+          for (int k = 0; k < C; k++) {
+            double inv = 1.0 / dcdmu[k][k];
+            dmu[k] = dc[k] * inv;
+            mo[k] += dmu[k];
+          }
+        } else {
+          /* general LU inverse; the reference's pow(x,-1) for 1/x is
+             reproduced by the division below only where pow is correctly rounded */
+          double a16[16], i16[16];
+          for (int k = 0; k < C; k++)
+            for (int l = 0; l < C; l++) a16[k * C + l] = dcdmu[k][l];
+          extern void gpo_matinv(double *, double *, int);
+          gpo_matinv(a16, i16, C);
+          for (int k = 0; k < C; k++) {
+            double s = 0;
+            for (int l = 0; l < C; l++) s = s + i16[k * C + l] * dc[l];
+            dmu[k] = s;
+          }
+          for (int k = 0; k < C; k++) mo[k] = dmu[k] + m[k];
+        }
+        for (int k = 0; k < C; k++) AT(g, mu_o, k, x, y) = mo[k];
+      }
+      for (int b = 0; b < P; b++) AT(g, phi_o, b, x, y) = ph[b] + dp[b];
+    }
+}
+
+/* LU inverse with partial pivoting of matrix.h:50-114 (same sequence, pow(x,-1)) */
+void gpo_matinv(double *a, double *inv, int n) {
+  double fac[16] = {0}, inv1[16], prod[16], v1[4], v[4];
+  int tag[4];
+  for (int k = 0; k < n; k++) tag[k] = k;
+  for (int k = 0; k < n; k++) {
+    double big = fabs(a[k * n + k]);
+    int t1 = k;
+    for (int i = k + 1; i < n; i++)
+      if (fabs(a[i * n + k]) > big) { t1 = i; big = a[i * n + k]; }
+    int ts = tag[k]; tag[k] = tag[t1]; tag[t1] = ts;
+    for (int i = 0; i < n; i++) { double s = a[k * n + i]; a[k * n + i] = a[t1 * n + i]; a[t1 * n + i] = s; }
+    for (int i = 0; i < k; i++) { double s = fac[k * n + i]; fac[k * n + i] = fac[t1 * n + i]; fac[t1 * n + i] = s; }
+    for (int i = k + 1; i < n; i++) {
+      double f = -a[i * n + k] / a[k * n + k];
+      fac[i * n + k] = -f;
+      for (int j = k; j < n; j++) a[i * n + j] = f * a[k * n + j] + a[i * n + j];
+    }
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) { if (i == j) fac[i * n + j] = 1; if (j > i) fac[i * n + j] = 0; }
+  for (int col = 0; col < n; col++) {
+    v1[0] = (col == 0) ? 1.0 : 0.0;
+    for (int i = 1; i < n; i++) {
+      double s = 0;
+      for (int j = 0; j < i; j++) s = s - fac[i * n + j] * v1[j];
+      v1[i] = ((i == col) ? 1.0 : 0.0) + s;
+    }
+    v[n - 1] = v1[n - 1] * pow(a[(n - 1) * n + n - 1], -1);
+    for (int i = n - 2; i >= 0; i--) {
+      double s = 0;
+      for (int j = i + 1; j < n; j++) s = s - a[i * n + j] * v[j];
+      v[i] = (v1[i] + s) * pow(a[i * n + i], -1);
+    }
+    for (int j = 0; j < n; j++) inv1[j * n + col] = v[j];
+  }
+  for (int j = 0; j < n; j++)
+    for (int q = 0; q < n; q++) inv[q * n + tag[j]] = inv1[q * n + j];
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < n; i++) {
+      double s = 0;
+      for (int j = 0; j < n; j++) s = s + fac[k * n + j] * a[j * n + i];
+      prod[k * n + i] = s;
+    }
+  for (int j = 0; j < n; j++)
+    for (int q = 0; q < n; q++) a[tag[j] * n + q] = prod[j * n + q];
+}
+
+/* stage 6: ghost copies, X faces over all y, then Y faces over all x */
+int gpo_apply_bc(const gp_params *p, double *phi, double *mu) {
+  grid G = {p, p->mesh_x + 6, p->mesh_y + 6, 0, p->nphases, p->ncomp};
+  const grid *g = &G;
+  long pts[4][3] = {{2, 1, 0}, {g->rx - 3, g->rx - 2, g->rx - 1}, {g->ry - 3, g->ry - 2, g->ry - 1}, {2, 1, 0}};
+  for (int face = 0; face < 4; face++) {
+    for (int fld = 0; fld < 2; fld++) {
+      int ty = p->bc_type[fld][face];
+      if (ty != GP_BC_NEUMANN && ty != GP_BC_PERIODIC) continue;
+      double *arr = fld ? mu : phi;
+      int nf = fld ? g->C : g->P;
+      for (int j = 0; j < 3; j++) {
+        long from = p->bc_proxy[fld][face][j], to = pts[face][j];
+        for (int a = 0; a < nf; a++) {
+          if (face < 2)
+            for (long y = 0; y < g->ry; y++) AT(g, arr, a, to, y) = AT(g, arr, a, from, y);
+          else
+            for (long x = 0; x < g->rx; x++) AT(g, arr, a, x, to) = AT(g, arr, a, x, from);
+        }
+      }
+    }
+  }
+  return GP_OK;
+}
+
+static int alloc_work(const grid *g, work *w) {
+  size_t n = g->n;
+  int P = g->P, C = g->C;
+  memset(w, 0, sizeof(*w));
+  w->gX = calloc(n * P, 8); w->sX = calloc(n * P, 8); w->gY = calloc(n * P, 8);
+  w->sY = calloc(n * P, 8); w->cX = calloc(n * P, 8); w->cY = calloc(n * P, 8);
+  w->dphi = calloc(n * P, 8);
+  w->jX = calloc(n * C, 8); w->jY = calloc(n * C, 8);
+  w->gmX = calloc(n * C, 8); w->gmY = calloc(n * C, 8);
+  w->DmX = calloc(n * C * C, 8); w->DmY = calloc(n * C * C, 8);
+  return (w->gX && w->sX && w->gY && w->sY && w->cX && w->cY && w->dphi && w->jX && w->jY &&
+          w->gmX && w->gmY && w->DmX && w->DmY) ? 0 : -1;
+}
+
+static void free_work(work *w) {
+  free(w->gX); free(w->sX); free(w->gY); free(w->sY); free(w->cX); free(w->cY);
+  free(w->dphi); free(w->jX); free(w->jY); free(w->gmX); free(w->gmY); free(w->DmX); free(w->DmY);
+}
+
+static int check(const gp_params *p) {
+  if (!p || p->nphases < 2 || p->nphases > PM || p->ncomp < 1 || p->ncomp > CM) return GP_E_ARG;
+  if (p->mesh_x < 4 || p->mesh_y < 4) return GP_E_ARG;
+  for (int f = 0; f < 2; f++)
+    for (int fc = 0; fc < 4; fc++)
+      if (p->bc_type[f][fc] != GP_BC_NEUMANN && p->bc_type[f][fc] != GP_BC_PERIODIC) return GP_E_UNSUPPORTED;
+  return GP_OK;
+}
+
+int gpo_dphi(const gp_params *p, const double *phi, const double *mu, double *dphi) {
+  int rc = check(p);
+  if (rc) return rc;
+  grid G = {p, p->mesh_x + 6, p->mesh_y + 6, 0, p->nphases, p->ncomp};
+  G.n = (size_t)G.rx * G.ry;
+  work w;
+  if (alloc_work(&G, &w)) { free_work(&w); return GP_E_NOMEM; }
+  stage_gradients(&G, phi, &w);
+  stage_dphi(&G, phi, mu, &w);
+  memcpy(dphi, w.dphi, sizeof(double) * G.n * G.P);
+  free_work(&w);
+  return GP_OK;
+}
+
+int gpo_step(const gp_params *p, double *phi, double *mu, long nsteps, int nthreads) {
+  int rc = check(p);
+  if (rc) return rc;
+#ifdef _OPENMP
+  int saved = omp_get_max_threads();
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  grid G = {p, p->mesh_x + 6, p->mesh_y + 6, 0, p->nphases, p->ncomp};
+  G.n = (size_t)G.rx * G.ry;
+  work w;
+  double *phi_o = malloc(sizeof(double) * G.n * G.P), *mu_o = malloc(sizeof(double) * G.n * G.C);
+  if (alloc_work(&G, &w) || !phi_o || !mu_o) {
+    free_work(&w); free(phi_o); free(mu_o);
+    return GP_E_NOMEM;
+  }
+  for (long t = 0; t < nsteps; t++) {
+    memcpy(phi_o, phi, sizeof(double) * G.n * G.P);
+    memcpy(mu_o, mu, sizeof(double) * G.n * G.C);
+    stage_gradients(&G, phi, &w);
+    stage_dphi(&G, phi, mu, &w);
+    stage_mu_faces(&G, phi, mu, &w);
+    stage_fluxes(&G, phi, mu, &w);
+    stage_update(&G, phi, mu, &w, phi_o, mu_o);
+    memcpy(phi, phi_o, sizeof(double) * G.n * G.P);
+    memcpy(mu, mu_o, sizeof(double) * G.n * G.C);
+    gpo_apply_bc(p, phi, mu);
+  }
+  free_work(&w);
+  free(phi_o);
+  free(mu_o);
+#ifdef _OPENMP
+  omp_set_num_threads(saved);
+#endif
+  return GP_OK;
+}

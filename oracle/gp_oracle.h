@@ -1,0 +1,37 @@
+/*
+ * gp_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU binary64 restatement of the grand-potential 2-D explicit step of the
+ * MicroSim serial solver, used as the checker for the MI355X kernels and as the
+ * timed CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product never links or calls it.
+ *
+ * Fields are padded planar arrays with the reference's ghost layers:
+ *   phi[a][x][y], x < rows_x = MESH_X+6, y < rows_y = MESH_Y+6 (y fastest),
+ *   interior x,y in [3, MESH+2]  (solverloop/serialinfo_xy.h:7-16).
+ */
+#ifndef GP_ORACLE_H_
+#define GP_ORACLE_H_
+
+#include "../include/gpstep.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Ghost-layer copies of phi and mu (solverloop/boundary_serial.h:176-203). */
+int gpo_apply_bc(const gp_params *p, double *phi, double *mu);
+
+/* nsteps full steps in place (microsim_gp.c:103-106); ghosts are refreshed after
+   each step.  nthreads <= 0 uses OpenMP's default, 1 runs serially. */
+int gpo_step(const gp_params *p, double *phi, double *mu, long nsteps, int nthreads);
+
+/* Delta-phi of one phase sweep on the ring-extended region x,y in [2, rows-3]
+   (zero elsewhere), for tests (calculate_divergence_phasefield.h:4-62). */
+int gpo_dphi(const gp_params *p, const double *phi, const double *mu, double *dphi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
