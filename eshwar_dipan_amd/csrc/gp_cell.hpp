@@ -1,0 +1,720 @@
+// gp_cell.hpp -- per-cell and per-face arithmetic of the grand-potential 2-D step
+// for gfx950, templated on the phase count P, the independent-component count C
+// and the real type R.
+//
+// Every function evaluates its value with the same sequence of IEEE operations
+// as the reference serial solver (file:line cited per function), so the binary64
+// instantiation compiled with -ffp-contract=off reproduces the reference bit for
+// bit.  The binary32 instantiation runs the same sequence in float with the
+// divisions by constant spacings replaced by multiplications (Exact = false).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "gpstep.h"
+
+namespace gpk {
+
+constexpr int PM = GP_MAX_PHASES;
+constexpr int CM = GP_MAX_COMP;
+
+// Constant tables the kernels read (device memory; indices are compile-time
+// constants after unrolling, so the loads are scalar).
+template <class R>
+struct DevParams {
+  long plane;             // elements per field plane (rows_x * pitch)
+  int rx, ry, pitch;      // padded extents (MESH+6) and row pitch
+  int mesh_x, mesh_y;
+  int obstacle, anisotropy, mu_solver;
+  int yskip;              // padded y index MESH_Y-2 whose y-face antitrapping is skipped
+  int gsrc[2][2][6];      // [field][axis][ghost slot] -> padded source index (slots: 0,1,2,n-3,n-2,n-1)
+  R dx, dy, dt, two_dy;
+  R inv_dx, inv_dy, inv_two_dy;
+  R epsilon, inv_eps, tau;
+  R w_pref;               // 16/(pi*pi) (obstacle) or 9.0 (well)
+  R jscale;               // (-0.25*M_PI*epsilon) or (-1.0*epsilon)
+  R well_at;              // 0.5/sqrt(2.0)
+  R gamma[PM][PM];
+  R gamma_abc[PM][PM][PM];
+  R tau_ab[PM][PM];
+  R dab[PM][PM];
+  R rot[PM][PM][2][2];
+  R rot_inv[PM][PM][2][2];
+  R at_coef[PM][CM];      // 1.0 - D_a[k][k]/D_liq[k][k]
+  R dcm[PM][CM][CM][CM];  // Diffusivity[b][i][k]*cmu[b][k][j] at [b][i][j][k]
+  R A[PM][CM][CM];
+  R cmu[PM][CM][CM];
+  R Bsh[PM][CM];          // Beq + dBbdT*(T-Teq)
+  R B[PM][CM];
+  R Cf[PM];
+};
+
+template <class R>
+struct Ops;
+template <>
+struct Ops<double> {
+  static constexpr bool Exact = true;
+};
+template <>
+struct Ops<float> {
+  static constexpr bool Exact = false;
+};
+
+template <class R>
+__device__ __forceinline__ R div_dx(R v, const DevParams<R>& p) {
+  if constexpr (Ops<R>::Exact) return v / p.dx; else return v * p.inv_dx;
+}
+template <class R>
+__device__ __forceinline__ R div_dy(R v, const DevParams<R>& p) {
+  if constexpr (Ops<R>::Exact) return v / p.dy; else return v * p.inv_dy;
+}
+template <class R>
+__device__ __forceinline__ R div_2dy(R v, const DevParams<R>& p) {
+  if constexpr (Ops<R>::Exact) return v / p.two_dy; else return v * p.inv_two_dy;
+}
+
+// ---------------------------------------------------------------- thermodynamics
+
+// c_a,i(mu)  (functionF_01.h:30-37)
+template <int C, class R>
+__device__ __forceinline__ R c_mu(const DevParams<R>& p, const R* mu, int a, int i) {
+  R s = R(0);
+#pragma unroll
+  for (int k = 0; k < C; k++) s += p.cmu[a][i][k] * (mu[k] - p.Bsh[a][k]);
+  return s;
+}
+
+// Mu_a,i(c)  (functionF_01.h:19-29)
+template <int C, class R>
+__device__ __forceinline__ R Mu_of_c(const DevParams<R>& p, const R* c, int a, int i) {
+  R s = R(2.0) * p.A[a][i][i] * c[i] + p.Bsh[a][i];
+#pragma unroll
+  for (int j = 0; j < C; j++)
+    if (i != j) s += p.A[a][i][j] * c[j];
+  return s;
+}
+
+// f_a(c)  (functionF_01.h:5-18)
+template <int C, class R>
+__device__ __forceinline__ R free_energy(const DevParams<R>& p, const R* c, int a) {
+  R s = R(0);
+#pragma unroll
+  for (int i = 0; i < C; i++) {
+#pragma unroll
+    for (int j = 0; j < C; j++)
+      if (i <= j) s += p.A[a][i][j] * c[i] * c[j];
+    s += p.B[a][i] * c[i];
+  }
+  s += p.Cf[a];
+  return s;
+}
+
+// h_a(phi)  (functionH.h:24-36)
+template <int P, class R>
+__device__ __forceinline__ R hphi(const R* ph, int a) {
+  R s1 = R(0);
+  R s = R(3.0) * ph[a] * ph[a] - R(2.0) * ph[a] * ph[a] * ph[a];
+#pragma unroll
+  for (int b = 0; b < P; b++)
+#pragma unroll
+    for (int c = 0; c < P; c++)
+      if (b != a && c != a && b < c) s1 += ph[b] * ph[c];
+  s1 *= R(2.0) * ph[a];
+  return s + s1;
+}
+
+// d h_b / d phi_a  (functionH.h:1-23)
+template <int P, class R>
+__device__ __forceinline__ R dhphi(const R* ph, int b, int a) {
+  R s = R(0);
+  if (a == b) {
+    s = R(6.0) * ph[a] * (R(1.0) - ph[a]);
+#pragma unroll
+    for (int e = 0; e < P; e++)
+#pragma unroll
+      for (int f = 0; f < P; f++)
+        if (e != b && f != b && e < f) s += R(2.0) * ph[e] * ph[f];
+  } else {
+#pragma unroll
+    for (int e = 0; e < P; e++)
+      if (e != b && e != a) s += R(2.0) * ph[e];
+    s *= ph[b];
+  }
+  return s;
+}
+
+// psi_b(mu) = f_b(c_b(mu)) - mu.c_b(mu)  (functionF_01.h:41-57, the b-term of dpsi)
+template <int C, class R>
+__device__ __forceinline__ R psi_of_mu(const DevParams<R>& p, const R* mu, int b) {
+  R c[C];
+  R psi = R(0);
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    c[k] = c_mu<C>(p, mu, b, k);
+    psi -= mu[k] * c[k];
+  }
+  psi += free_energy<C>(p, c, b);
+  return psi;
+}
+
+// tau(phi)  (functionTau.h:1-16)
+template <int P, class R>
+__device__ __forceinline__ R tau_phi(const DevParams<R>& p, const R* ph) {
+  R s = R(0), s1 = R(0);
+#pragma unroll
+  for (int a = 0; a < P; a++)
+#pragma unroll
+    for (int b = 0; b < P; b++)
+      if (a < b) {
+        s += p.tau_ab[a][b] * ph[a] * ph[b];
+        s1 += ph[a] * ph[b];
+      }
+  if (s1 != R(0)) return s / s1;
+  return p.tau;
+}
+
+// ---------------------------------------------------------------- anisotropy
+
+// 2-D block of multiply(R, q) (matrix.h:22-33, DIMENSION = 2)
+template <class R>
+__device__ __forceinline__ void rot2(const R (&M)[2][2], const R* q, R* out) {
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    R s = R(0);
+    s = s + M[i][0] * q[0];
+    s = s + M[i][1] * q[1];
+    out[i] = s;
+  }
+}
+
+// a_c(q) of the four-fold cubic form  (anisotropy_01.h:29-48)
+template <class R>
+__device__ __forceinline__ R ac_of_q(R dab, const R* q) {
+  R qx2 = q[0] * q[0], qx4 = qx2 * qx2, qy2 = q[1] * q[1], qy4 = qy2 * qy2;
+  R q2 = qx2 + qy2;
+  if (fabs(q2) > R(1.0e-15)) return R(1.0) - dab * (R(3.0) - R(4.0) * (qx4 + qy4) / (q2 * q2));
+  return R(1.0);
+}
+
+// d a_c / d q  (anisotropy_01.h:9-27)
+template <class R>
+__device__ __forceinline__ void dadq_of_q(R dab, const R* q, R* d) {
+  R qx3 = q[0] * q[0] * q[0], qy3 = q[1] * q[1] * q[1];
+  R q2 = q[0] * q[0] + q[1] * q[1];
+  R q22 = q2 * q2, q23 = q2 * q2 * q2;
+  R q4 = qx3 * q[0] + qy3 * q[1];
+  if (fabs(q2) > R(1.0e-15)) {
+    d[0] = R(16.0) * dab * (qx3 / q22 - q[0] * q4 / q23);
+    d[1] = R(16.0) * dab * (qy3 / q22 - q[1] * q4 / q23);
+  } else {
+    d[0] = R(0);
+    d[1] = R(0);
+  }
+}
+
+// ---------------------------------------------------------------- neighbourhood
+
+// phi gradients around one cell, from the 3x3 block v[a][i][j] (i,j = 0..2 for
+// offsets -1..+1); the formulas are calculate_gradients.h:108-128.
+template <int P, class R>
+struct Nbhd {
+  R ph[P];                 // phi at the cell
+  R gXc[P], gXb[P];        // x-face gradients at faces (x|x+1), (x-1|x)
+  R sXc[P], sXb[P];        // x-face averages
+  R gYc[P], gYl[P];        // y-face gradients at faces (y|y+1), (y-1|y)
+  R sYc[P], sYl[P];
+  R cXc[P], cYc[P];        // centred gradients at the cell
+  R cYf[P], cYb[P];        // centred y-gradients at (x+1,y), (x-1,y)
+  R cXr[P], cXl[P];        // centred x-gradients at (x,y+1), (x,y-1)
+};
+
+template <int P, class R>
+__device__ __forceinline__ void make_nbhd(const DevParams<R>& p, const R (&v)[P][3][3], Nbhd<P, R>& n) {
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    n.ph[a] = v[a][1][1];
+    n.gXc[a] = div_dx(v[a][2][1] - v[a][1][1], p);
+    n.gXb[a] = div_dx(v[a][1][1] - v[a][0][1], p);
+    n.sXc[a] = R(0.5) * (v[a][2][1] + v[a][1][1]);
+    n.sXb[a] = R(0.5) * (v[a][1][1] + v[a][0][1]);
+    n.gYc[a] = div_dy(v[a][1][2] - v[a][1][1], p);
+    n.gYl[a] = div_dy(v[a][1][1] - v[a][1][0], p);
+    n.sYc[a] = R(0.5) * (v[a][1][2] + v[a][1][1]);
+    n.sYl[a] = R(0.5) * (v[a][1][1] + v[a][1][0]);
+    n.cXc[a] = R(0.5) * (n.gXc[a] + n.gXb[a]);
+    n.cYc[a] = div_2dy(v[a][1][2] - v[a][1][0], p);
+    n.cYf[a] = div_2dy(v[a][2][2] - v[a][2][0], p);
+    n.cYb[a] = div_2dy(v[a][0][2] - v[a][0][0], p);
+    R gx_r = div_dx(v[a][2][2] - v[a][1][2], p), gx_rb = div_dx(v[a][1][2] - v[a][0][2], p);
+    R gx_l = div_dx(v[a][2][0] - v[a][1][0], p), gx_lb = div_dx(v[a][1][0] - v[a][0][0], p);
+    n.cXr[a] = R(0.5) * (gx_r + gx_rb);
+    n.cXl[a] = R(0.5) * (gx_l + gx_lb);
+  }
+}
+
+// ---------------------------------------------------------------- delta-phi
+
+// dA/dphi_a  (function_A_00.h:4-24 isotropic; function_A_01.h:4-71 anisotropic)
+template <int P, class R>
+__device__ __forceinline__ R dAdphi(const DevParams<R>& p, const Nbhd<P, R>& n, int a) {
+  if (!p.anisotropy) {
+    R sum = R(0);
+#pragma unroll
+    for (int b = 0; b < P; b++) {
+      if (b == a) continue;
+      R sp = R(0);
+      sp += (n.ph[a] * n.cXc[b] - n.ph[b] * n.cXc[a]) * n.cXc[b];
+      sp += (n.ph[a] * n.cYc[b] - n.ph[b] * n.cYc[a]) * n.cYc[b];
+      sp *= p.gamma[a][b];
+      sum += sp;
+    }
+    return R(2.0) * sum;
+  }
+  R sum1 = R(0), sum2 = R(0);
+#pragma unroll
+  for (int b = 0; b < P; b++) {
+    if (b == a) continue;
+    R u[2], q[2], d[2], dq[2], rdq[2];
+    u[0] = (n.ph[a] * n.cXc[b] - n.ph[b] * n.cXc[a]);
+    u[1] = (n.ph[a] * n.cYc[b] - n.ph[b] * n.cYc[a]);
+    rot2(p.rot[a][b], u, q);
+    dadq_of_q(p.dab[a][b], q, d);
+    R qab2 = (q[0] * q[0] + q[1] * q[1]);
+    R ac1 = ac_of_q(p.dab[a][b], q), ac2 = ac1 * ac1;
+    dq[0] = n.cXc[b];
+    dq[1] = n.cYc[b];
+    rot2(p.rot[a][b], dq, rdq);
+    R s1 = R(0), s2 = R(0);
+    s1 += q[0] * rdq[0];
+    s1 += q[1] * rdq[1];
+    s2 += d[0] * rdq[0];
+    s2 += d[1] * rdq[1];
+    s1 *= p.gamma[a][b] * ac2;
+    s2 *= p.gamma[a][b] * ac1 * qab2;
+    sum1 += s1;
+    sum2 += s2;
+  }
+  return R(2.0) * (sum1 + sum2);
+}
+
+// per-face data of the anisotropic divergence for pair (a,b)
+template <class R>
+struct FaceQ {
+  R flux;      // sa*gb - sb*ga (unrotated normal component)
+  R ac, ac2;   // a_c(q), a_c^2
+  R q2;        // |q|^2
+  R rd[2];     // R^-1 * da_c/dq
+};
+
+template <class R>
+__device__ __forceinline__ void face_q(const DevParams<R>& p, int a, int b, R u0, R u1, R flux, FaceQ<R>& f) {
+  R u[2] = {u0, u1}, q[2], d[2];
+  rot2(p.rot[a][b], u, q);
+  f.flux = flux;
+  f.ac = ac_of_q(p.dab[a][b], q);
+  f.ac2 = f.ac * f.ac;
+  dadq_of_q(p.dab[a][b], q, d);
+  rot2(p.rot_inv[a][b], d, f.rd);
+  f.q2 = q[0] * q[0] + q[1] * q[1];
+}
+
+// div (dA / d grad phi_a)  (function_A_00.h:25-58 isotropic; function_A_01.h:101-219 anisotropic)
+template <int P, class R>
+__device__ __forceinline__ R divdAdgradphi(const DevParams<R>& p, const Nbhd<P, R>& n, int a) {
+  if (!p.anisotropy) {
+    R sum = R(0);
+#pragma unroll
+    for (int b = 0; b < P; b++) {
+      if (b == a) continue;
+      R fc = n.sXc[a] * n.gXc[b] - n.sXc[b] * n.gXc[a];
+      R fb = n.sXb[a] * n.gXb[b] - n.sXb[b] * n.gXb[a];
+      R dv = div_dx(fc * (-n.sXc[b]) - (fb * (-n.sXb[b])), p);
+      R gc = n.sYc[a] * n.gYc[b] - n.sYc[b] * n.gYc[a];
+      R gl = n.sYl[a] * n.gYl[b] - n.sYl[b] * n.gYl[a];
+      dv += div_dy(gc * (-n.sYc[b]) - (gl * (-n.sYl[b])), p);
+      dv *= p.gamma[a][b];
+      sum += dv;
+    }
+    return R(2.0) * sum;
+  }
+  R sum1 = R(0), sum2 = R(0);
+#pragma unroll
+  for (int b = 0; b < P; b++) {
+    if (b == a) continue;
+    FaceQ<R> F, Bk, Rt, L;
+    face_q(p, a, b, n.sXc[a] * n.gXc[b] - n.sXc[b] * n.gXc[a],
+           R(0.5) * (n.sXc[a] * (n.cYc[b] + n.cYf[b]) - n.sXc[b] * (n.cYc[a] + n.cYf[a])),
+           n.sXc[a] * n.gXc[b] - n.sXc[b] * n.gXc[a], F);
+    face_q(p, a, b, n.sXb[a] * n.gXb[b] - n.sXb[b] * n.gXb[a],
+           R(0.5) * (n.sXb[a] * (n.cYb[b] + n.cYc[b]) - n.sXb[b] * (n.cYb[a] + n.cYc[a])),
+           n.sXb[a] * n.gXb[b] - n.sXb[b] * n.gXb[a], Bk);
+    face_q(p, a, b, R(0.5) * (n.sYc[a] * (n.cXc[b] + n.cXr[b]) - n.sYc[b] * (n.cXc[a] + n.cXr[a])),
+           n.sYc[a] * n.gYc[b] - n.sYc[b] * n.gYc[a],
+           n.sYc[a] * n.gYc[b] - n.sYc[b] * n.gYc[a], Rt);
+    face_q(p, a, b, R(0.5) * (n.sYl[a] * (n.cXl[b] + n.cXc[b]) - n.sYl[b] * (n.cXl[a] + n.cXc[a])),
+           n.sYl[a] * n.gYl[b] - n.sYl[b] * n.gYl[a],
+           n.sYl[a] * n.gYl[b] - n.sYl[b] * n.gYl[a], L);
+    R d1 = div_dx((F.ac2 * F.flux * (-n.sXc[b])) - (Bk.ac2 * Bk.flux * (-n.sXb[b])), p);
+    d1 += div_dy((Rt.ac2 * Rt.flux * (-n.sYc[b])) - (L.ac2 * L.flux * (-n.sYl[b])), p);
+    d1 *= p.gamma[a][b];
+    sum1 += d1;
+    R d2 = div_dx(F.ac * F.q2 * F.rd[0] * (-n.sXc[b]) - Bk.ac * Bk.q2 * Bk.rd[0] * (-n.sXb[b]), p);
+    d2 += div_dy(Rt.ac * Rt.q2 * Rt.rd[1] * (-n.sYc[b]) - L.ac * L.q2 * L.rd[1] * (-n.sYl[b]), p);
+    d2 *= p.gamma[a][b];
+    sum2 += d2;
+  }
+  return (R(2.0) * (sum1 + sum2) + R(0));
+}
+
+// dw/dphi_a  (functionW_01.h:6-39 obstacle, functionW_02.h:4-33 well)
+template <int P, class R>
+__device__ __forceinline__ R dwdphi(const DevParams<R>& p, const R* ph, const bool* act, int a) {
+  R sum = R(0);
+  if (p.obstacle) {
+#pragma unroll
+    for (int b = 0; b < P; b++)
+      if (b != a && act[b]) {
+        if ((ph[a] * ph[b]) >= R(0)) sum += p.gamma[a][b] * (ph[b]);
+        else sum += -p.gamma[a][b] * (ph[b]);
+      }
+  } else {
+#pragma unroll
+    for (int b = 0; b < P; b++)
+      if (b != a && act[b]) sum += R(2.0) * p.gamma[a][b] * (ph[b] * ph[b] * ph[a]);
+  }
+  sum *= p.w_pref;
+#pragma unroll
+  for (int b = 0; b < P; b++)
+#pragma unroll
+    for (int c = 0; c < P; c++)
+      if (b != a && c != a && b < c && act[b] && act[c]) {
+        R pbc = ph[b] * ph[c];
+        if (ph[a] * pbc >= R(0)) sum += p.gamma_abc[a][b][c] * pbc;
+        else sum += -p.gamma_abc[a][b][c] * pbc;
+      }
+  return sum;
+}
+
+// Gibbs-simplex projection  (simplex_projection.h:1-49)
+template <int P, class R>
+__device__ __forceinline__ void project_simplex(const R* ph, const bool* act, R* dp) {
+  R Deltaphi = R(0), sum_phib = R(0);
+  int count = 0;
+#pragma unroll
+  for (int a = 0; a < P; a++)
+    if (act[a] && (ph[a] + dp[a]) > R(0) && (ph[a] + dp[a]) < R(1.0)) {
+      count++;
+      sum_phib += ph[a];
+    }
+#pragma unroll
+  for (int a = 0; a < P; a++)
+    if ((ph[a] + dp[a]) < R(0)) {
+      R Da = fabs(dp[a] + ph[a]);
+      dp[a] += Da;
+      Deltaphi += Da;
+    }
+  if (Deltaphi > R(0)) {
+#pragma unroll
+    for (int b = 0; b < P; b++)
+      if (act[b] && ((ph[b] + dp[b]) > R(0)) && ((ph[b] + dp[b]) < R(1.0))) {
+        if (fabs(sum_phib) > R(0)) dp[b] -= Deltaphi * (ph[b]) / (sum_phib);
+        else dp[b] -= Deltaphi / R(count);
+      }
+  }
+  bool done = false;
+#pragma unroll
+  for (int a = 0; a < P; a++)
+    if (!done && act[a] && (ph[a] + dp[a]) > R(1.0)) {
+      dp[a] = (R(1.0) - ph[a]);
+#pragma unroll
+      for (int b = 0; b < P; b++)
+        if (act[b] && b != a) dp[b] = -ph[b];
+      done = true;
+    }
+}
+
+// delta-phi of one cell  (calculate_divergence_phasefield.h:4-62)
+template <int P, int C, class R>
+__device__ __forceinline__ void cell_dphi(const DevParams<R>& p, const Nbhd<P, R>& n, const R* mu, R* dp) {
+  R divphi[P], lam[P], psi[P];
+  bool act[P];
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    divphi[a] = div_dx(n.gXc[a] - n.gXb[a], p);
+    divphi[a] += div_dy(n.gYc[a] - n.gYl[a], p);
+    act[a] = fabs(divphi[a]) > R(0);
+  }
+  bool any = false;
+#pragma unroll
+  for (int a = 0; a < P; a++) any = any || act[a];
+#pragma unroll
+  for (int a = 0; a < P; a++) dp[a] = R(0);
+  if (any) {
+#pragma unroll
+    for (int b = 0; b < P; b++) psi[b] = psi_of_mu<C>(p, mu, b);
+    R sl = R(0);
+    int active = 0;
+#pragma unroll
+    for (int a = 0; a < P; a++)
+      if (act[a]) {
+        R dps = R(0);
+#pragma unroll
+        for (int b = 0; b < P; b++) dps += psi[b] * dhphi<P>(n.ph, b, a);
+        lam[a] = p.epsilon * (-dAdphi<P>(p, n, a) + divdAdgradphi<P>(p, n, a)) -
+                 p.inv_eps * dwdphi<P>(p, n.ph, act, a) - dps;
+        sl += lam[a];
+        active++;
+      }
+    sl /= R(active);
+    R tau = tau_phi<P>(p, n.ph);
+#pragma unroll
+    for (int a = 0; a < P; a++)
+      if (act[a]) dp[a] = p.dt * (lam[a] - sl) / (tau * p.epsilon);
+  }
+  // the projection also runs where no phase is active: it lifts negative phi
+  project_simplex<P>(n.ph, act, dp);
+}
+
+// ---------------------------------------------------------------- mu update
+
+// s(phi) of the obstacle antitrapping  (calculate_fluxes_concentration.h:42-59)
+template <int P, class R>
+__device__ __forceinline__ R s_phi(const R* ph, int a) {
+  R q = ph[a] * (R(1.0) - ph[a]);
+  if (q > R(0)) return ph[a] * (R(1.0) - hphi<P>(ph, a)) / (sqrt(q));
+  return R(0);
+}
+
+// D_ij of one cell  (functionD.h:1-15)
+template <int P, int C, class R>
+__device__ __forceinline__ void cell_D(const DevParams<R>& p, const R* ph, R (&D)[C][C]) {
+#pragma unroll
+  for (int i = 0; i < C; i++)
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      R s = R(0);
+#pragma unroll
+      for (int b = 0; b < P; b++)
+#pragma unroll
+        for (int k = 0; k < C; k++) s += p.dcm[b][i][j][k] * ph[b];
+      D[i][j] = s;
+    }
+}
+
+// Antitrapping flux on one face between cell p (x or y) and its +1 neighbour q
+// (calculate_fluxes_concentration.h:25-142).  g_p: normal gradients at the face,
+// ct_p / ct_q: tangential centred gradients at p and q (the face's tangential
+// gradient is their mean), phase fields, potentials and delta-phi of both cells.
+template <int P, int C, class R>
+__device__ __forceinline__ void face_jat(const DevParams<R>& p, const R* g_p, const R* ct_p, const R* ct_q,
+                                         const R* ph_p, const R* ph_q, const R* mu_p, const R* mu_q,
+                                         const R* dp_p, const R* dp_q, bool enabled, R* jat) {
+  constexpr int L = P - 1;
+#pragma unroll
+  for (int k = 0; k < C; k++) jat[k] = R(0);
+  if (!enabled) return;
+  R tl = R(0.5) * (ct_p[L] + ct_q[L]);
+  R ngl = sqrt(tl * tl + g_p[L] * g_p[L]);
+#pragma unroll
+  for (int a = 0; a < L; a++) {
+    R t = R(0.5) * (ct_p[a] + ct_q[a]);
+    R ngp = sqrt(t * t + g_p[a] * g_p[a]);
+    if (ngp != R(0)) {
+      R s_q = R(0), s_p = R(0);
+      if (p.obstacle) {
+        s_q = s_phi<P>(ph_q, a);
+        s_p = s_phi<P>(ph_p, a);
+      }
+      R sp = g_p[a] * g_p[L];
+      sp += t * tl;
+      if (ngl > R(0)) sp /= (ngl * ngp);
+#pragma unroll
+      for (int k = 0; k < C; k++) {
+        R clq = c_mu<C>(p, mu_q, L, k), clp = c_mu<C>(p, mu_p, L, k);
+        R csq = c_mu<C>(p, mu_q, a, k), csp = c_mu<C>(p, mu_p, a, k);
+        if (!p.obstacle)
+          jat[k] += p.at_coef[a][k] * fabs(sp) * p.well_at * ((clq - csq) * dp_q[a] + (clp - csp) * dp_p[a]) * g_p[a] / ngp;
+        else
+          jat[k] += p.at_coef[a][k] * fabs(sp) * R(0.5) * (s_q * (clq - csq) * dp_q[a] + s_p * (clp - csp) * dp_p[a]) * g_p[a] / ngp;
+      }
+    }
+  }
+}
+
+// 1/x for the general LU solve.  The reference uses pow(x,-1) (matrix.h:121,127),
+// which glibc evaluates correctly rounded for these arguments, i.e. equal to 1/x.
+template <class R>
+__device__ __forceinline__ R recip(R x) { return R(1.0) / x; }
+
+// LU inverse with partial pivoting of matrix.h:50-114 (general mu solve)
+template <int C, class R>
+__device__ __forceinline__ void lu_inverse(R (&a)[C][C], R (&inv)[C][C]) {
+  R fac[C][C], inv1[C][C];
+  int tag[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) tag[k] = k;
+#pragma unroll
+  for (int i = 0; i < C; i++)
+#pragma unroll
+    for (int j = 0; j < C; j++) fac[i][j] = R(0);
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    R big = fabs(a[k][k]);
+    int t1 = k;
+#pragma unroll
+    for (int i = k + 1; i < C; i++)
+      if (fabs(a[i][k]) > big) { t1 = i; big = a[i][k]; }
+    if (t1 != k) {
+      int ts = tag[k]; tag[k] = tag[t1]; tag[t1] = ts;
+#pragma unroll
+      for (int i = 0; i < C; i++) {
+#pragma unroll
+        for (int r = k + 1; r < C; r++)
+          if (r == t1) { R s = a[k][i]; a[k][i] = a[r][i]; a[r][i] = s; }
+      }
+#pragma unroll
+      for (int i = 0; i < k; i++) {
+#pragma unroll
+        for (int r = k + 1; r < C; r++)
+          if (r == t1) { R s = fac[k][i]; fac[k][i] = fac[r][i]; fac[r][i] = s; }
+      }
+    }
+#pragma unroll
+    for (int i = k + 1; i < C; i++) {
+      R f = -a[i][k] / a[k][k];
+      fac[i][k] = -f;
+#pragma unroll
+      for (int j = k; j < C; j++) a[i][j] = f * a[k][j] + a[i][j];
+    }
+  }
+#pragma unroll
+  for (int col = 0; col < C; col++) {
+    R v1[C], v[C];
+    v1[0] = (col == 0) ? R(1.0) : R(0);
+#pragma unroll
+    for (int i = 1; i < C; i++) {
+      R s = R(0);
+#pragma unroll
+      for (int j = 0; j < i; j++) s = s - fac[i][j] * v1[j];
+      v1[i] = ((i == col) ? R(1.0) : R(0)) + s;
+    }
+    v[C - 1] = v1[C - 1] * recip(a[C - 1][C - 1]);
+#pragma unroll
+    for (int i = C - 2; i >= 0; i--) {
+      R s = R(0);
+#pragma unroll
+      for (int j = i + 1; j < C; j++) s = s - a[i][j] * v[j];
+      v[i] = (v1[i] + s) * recip(a[i][i]);
+    }
+#pragma unroll
+    for (int j = 0; j < C; j++) inv1[j][col] = v[j];
+  }
+#pragma unroll
+  for (int j = 0; j < C; j++)
+#pragma unroll
+    for (int q = 0; q < C; q++)
+#pragma unroll
+      for (int t = 0; t < C; t++)
+        if (tag[j] == t) inv[q][t] = inv1[q][j];
+}
+
+// mu and phi update of one cell from its face fluxes
+// (calculate_divergence_concentration.h:4-153).  jat*/Dm*/gm*: antitrapping flux,
+// face mobility and mu gradient on the faces front (x|x+1), back (x-1|x),
+// right (y|y+1), left (y-1|y).
+template <int P, int C, class R>
+__device__ __forceinline__ void cell_update(const DevParams<R>& p, const R* ph, const R* mu, const R* dp,
+                                            const R* jf, const R* jb, const R* jr, const R* jl,
+                                            const R (&Dmf)[C][C], const R (&Dmb)[C][C],
+                                            const R (&Dmr)[C][C], const R (&Dml)[C][C],
+                                            const R* gmf, const R* gmb, const R* gmr, const R* gml,
+                                            R* ph_o, R* mu_o) {
+  R divflux[C], divjat[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    divflux[k] = R(0);
+#pragma unroll
+    for (int l = 0; l < C; l++) {
+      divflux[k] += div_dx(Dmf[k][l] * gmf[l] - Dmb[k][l] * gmb[l], p);
+      divflux[k] += div_dy(Dmr[k][l] * gmr[l] - Dml[k][l] * gml[l], p);
+    }
+    divjat[k] = div_dx((jf[k] - jb[k]) * p.jscale, p);
+    divjat[k] += div_dy((jr[k] - jl[k]) * p.jscale, p);
+  }
+  int bulk = -1;
+#pragma unroll
+  for (int a = P - 1; a >= 0; a--)
+    if (ph[a] == R(1.0)) bulk = a;
+  if (bulk >= 0) {
+    R cn[C];
+#pragma unroll
+    for (int a = 0; a < P; a++)
+      if (a == bulk) {
+#pragma unroll
+        for (int k = 0; k < C; k++) cn[k] = c_mu<C>(p, mu, a, k) + p.dt * (divflux[k]) - divjat[k];
+#pragma unroll
+        for (int k = 0; k < C; k++) mu_o[k] = Mu_of_c<C>(p, cn, a, k);
+      }
+  } else {
+    R sum[C], dc[C], dcdmu[C][C], h[P];
+#pragma unroll
+    for (int k = 0; k < C; k++) sum[k] = R(0);
+#pragma unroll
+    for (int a = 0; a < P; a++) {
+      R sdh = R(0);
+#pragma unroll
+      for (int b = 0; b < P; b++) sdh += dhphi<P>(ph, a, b) * dp[b];
+#pragma unroll
+      for (int k = 0; k < C; k++) sum[k] += c_mu<C>(p, mu, a, k) * sdh;
+      h[a] = hphi<P>(ph, a);
+    }
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      dc[k] = p.dt * (divflux[k]) - divjat[k] - sum[k];
+#pragma unroll
+      for (int l = 0; l < C; l++) {
+        dcdmu[k][l] = R(0);
+#pragma unroll
+        for (int a = 0; a < P; a++) dcdmu[k][l] += p.cmu[a][k][l] * h[a];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < C; k++) mu_o[k] = mu[k];
+    if (p.mu_solver == GP_MU_BINARY) {
+      R inv = R(1.0) / dcdmu[0][0];
+      mu_o[0] = mu[0] + dc[0] * inv;
+    } else if (C == 2 && p.mu_solver == GP_MU_TERNARY) {
+      R DET = dcdmu[0][0] * dcdmu[C - 1][C - 1] - dcdmu[0][C - 1] * dcdmu[C - 1][0];
+      R i00 = dcdmu[C - 1][C - 1] / DET, i11 = dcdmu[0][0] / DET;
+      R i01 = -dcdmu[0][C - 1] / DET, i10 = -dcdmu[C - 1][0] / DET;
+      R d0 = R(0), d1 = R(0);
+      d0 += i00 * dc[0];
+      d0 += i01 * dc[C - 1];
+      d1 += i10 * dc[0];
+      d1 += i11 * dc[C - 1];
+      mu_o[0] = d0 + mu[0];
+      mu_o[C - 1] = d1 + mu[C - 1];
+    } else if (p.mu_solver == GP_MU_DILUTE) {
+#pragma unroll
+      for (int k = 0; k < C; k++) {
+        R inv = R(1.0) / dcdmu[k][k];
+        mu_o[k] = mu[k] + dc[k] * inv;
+      }
+    } else {
+      R inv[C][C];
+      lu_inverse<C>(dcdmu, inv);
+#pragma unroll
+      for (int k = 0; k < C; k++) {
+        R s = R(0);
+#pragma unroll
+        for (int l = 0; l < C; l++) s = s + inv[k][l] * dc[l];
+        mu_o[k] = s + mu[k];
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < P; b++) ph_o[b] = ph[b] + dp[b];
+}
+
+}  // namespace gpk

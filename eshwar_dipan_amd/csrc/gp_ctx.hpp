@@ -1,0 +1,44 @@
+// gp_ctx.hpp -- the opaque context behind gpstep.h and the per-(P, C, real type)
+// entry points that the instantiation files provide.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gpstep.h"
+
+struct gp_ctx {
+  gp_params hp;         // host copy of the parameters
+  int dtype;            // GP_F64 / GP_F32
+  int P, C;
+  int rx, ry, pitch;
+  long plane;           // elements per plane
+  size_t esize;         // bytes per element
+  int device;
+  void *phi[2], *mu[2]; // ping-pong field buffers (padded planes)
+  void *mulo[2];        // binary32 path: low-order part of mu (compensated update)
+  void *dphi;           // two-pass path: delta-phi planes
+  void *dprm;           // DevParams<R> on the device
+  double *dpart;        // stats partials
+  int nstat_blocks;
+  int cur;              // index of the current state buffers
+  int path;             // 0 fused, 1 two-pass
+  hipStream_t stream;
+  hipEvent_t ev0, ev1;
+  double last_ms;
+};
+
+namespace gpk {
+
+// per-(P,C,R) entry points, instantiated in gp_inst_p*.hip
+template <int P, int C, class R>
+int run_steps(gp_ctx *c, long nsteps);
+template <int P, int C, class R>
+int run_stats(gp_ctx *c, gp_stats *st);
+
+}  // namespace gpk
+
+#define GP_HIP_CHECK(expr)                 \
+  do {                                     \
+    hipError_t e_ = (expr);                \
+    if (e_ != hipSuccess) return GP_E_HIP; \
+  } while (0)

@@ -1,0 +1,9 @@
+// gp_inst_p9.hip -- kernels for P = 9 phases; compiled once per real type
+// (-DGP_REAL=double with -ffp-contract=off, -DGP_REAL=float with -ffp-contract=fast).
+#include "gp_run.hpp"
+
+#ifndef GP_REAL
+#error GP_REAL must be double or float
+#endif
+
+GP_INSTANTIATE(9, 1, GP_REAL)

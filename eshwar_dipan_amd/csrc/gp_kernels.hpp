@@ -1,0 +1,265 @@
+// gp_kernels.hpp -- gfx950 kernels of the grand-potential 2-D step.
+//
+// Field storage: one plane per phase / component, rows_x = MESH_X+6 rows of
+// `pitch` elements (y fastest, pitch >= MESH_Y+6 rounded to 256 bytes), i.e. the
+// reference's padded grid with its three ghost layers (solverloop/serialinfo_xy.h:7-16).
+//
+// Two-pass path (parity reference kernels):
+//   k_phase : delta-phi on x,y in [2, rows-3] (interior plus the ghost ring the
+//             antitrapping fluxes read)        -- solverloop_phasefield
+//   k_conc  : mu and phi update of the interior, written out-of-place, each cell
+//             also written to the ghost cells that copy it (so the ghost copies of
+//             apply_boundary_conditions need no separate pass) -- solverloop_concentration
+// Fused path (one launch per step): see k_fused in gp_fused.hpp.
+#pragma once
+
+#include "gp_cell.hpp"
+
+namespace gpk {
+
+constexpr int BX = 64;  // threads along y (contiguous)
+constexpr int BY = 4;   // threads along x
+
+template <class R>
+__device__ __forceinline__ size_t idx(const DevParams<R>& p, int a, int x, int y) {
+  return (size_t)a * p.plane + (size_t)x * p.pitch + y;
+}
+
+// Padded ghost indices of slot s (0..5) along an axis of n padded rows.
+__device__ __forceinline__ int ghost_index(int s, int n) { return s < 3 ? s : n - 6 + s; }
+
+// Write value v of field plane `base` at interior (x,y) and at every ghost cell
+// whose copy source is (x,y) for field f (X faces first then Y faces in the
+// reference, so a corner ghost holds the value at (proxy_x, proxy_y)).
+template <class R>
+__device__ __forceinline__ void write_with_ghosts(const DevParams<R>& p, int f, R* __restrict__ base, int x, int y, R v) {
+  int tx[3], ty[3], nx = 1, ny = 1;
+  tx[0] = x;
+  ty[0] = y;
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    if (p.gsrc[f][0][s] == x) tx[nx++] = ghost_index(s, p.rx);
+    if (p.gsrc[f][1][s] == y) ty[ny++] = ghost_index(s, p.ry);
+  }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) base[(size_t)tx[i] * p.pitch + ty[j]] = v;
+}
+
+template <int P, class R>
+__device__ __forceinline__ void load3x3(const DevParams<R>& p, const R* __restrict__ phi, int x, int y, R (&v)[P][3][3]) {
+#pragma unroll
+  for (int a = 0; a < P; a++)
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) v[a][i][j] = phi[idx(p, a, x + i - 1, y + j - 1)];
+}
+
+template <int P, int C, class R>
+__global__ void __launch_bounds__(BX * BY) k_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                                   const R* __restrict__ mu, R* __restrict__ dphi) {
+  const DevParams<R>& p = *prm;
+  const int y = 2 + blockIdx.x * BX + threadIdx.x;
+  const int x = 2 + blockIdx.y * BY + threadIdx.y;
+  if (x > p.rx - 3 || y > p.ry - 3) return;
+  R v[P][3][3];
+  load3x3<P>(p, phi, x, y, v);
+  Nbhd<P, R> n;
+  make_nbhd<P>(p, v, n);
+  R m[C], dp[P];
+#pragma unroll
+  for (int k = 0; k < C; k++) m[k] = mu[idx(p, k, x, y)];
+  cell_dphi<P, C>(p, n, m, dp);
+#pragma unroll
+  for (int a = 0; a < P; a++) dphi[idx(p, a, x, y)] = dp[a];
+}
+
+template <int P, int C, class R>
+__global__ void __launch_bounds__(BX * BY) k_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                                  const R* __restrict__ mu, const R* __restrict__ dphi,
+                                                  R* __restrict__ phi_o, R* __restrict__ mu_o) {
+  const DevParams<R>& p = *prm;
+  const int y = 3 + blockIdx.x * BX + threadIdx.x;
+  const int x = 3 + blockIdx.y * BY + threadIdx.y;
+  if (x > p.mesh_x + 2 || y > p.mesh_y + 2) return;
+  R v[P][3][3];
+  load3x3<P>(p, phi, x, y, v);
+  Nbhd<P, R> n;
+  make_nbhd<P>(p, v, n);
+  R phc[P], phf[P], phb[P], phr[P], phl[P];
+  R dc[P], df[P], db[P], dr[P], dl[P];
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    phc[a] = v[a][1][1];
+    phf[a] = v[a][2][1];
+    phb[a] = v[a][0][1];
+    phr[a] = v[a][1][2];
+    phl[a] = v[a][1][0];
+    dc[a] = dphi[idx(p, a, x, y)];
+    df[a] = dphi[idx(p, a, x + 1, y)];
+    db[a] = dphi[idx(p, a, x - 1, y)];
+    dr[a] = dphi[idx(p, a, x, y + 1)];
+    dl[a] = dphi[idx(p, a, x, y - 1)];
+  }
+  R mc[C], mf[C], mb[C], mr[C], ml[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    mc[k] = mu[idx(p, k, x, y)];
+    mf[k] = mu[idx(p, k, x + 1, y)];
+    mb[k] = mu[idx(p, k, x - 1, y)];
+    mr[k] = mu[idx(p, k, x, y + 1)];
+    ml[k] = mu[idx(p, k, x, y - 1)];
+  }
+  // antitrapping on the four faces
+  R jf[C], jb[C], jr[C], jl[C];
+  face_jat<P, C>(p, n.gXc, n.cYc, n.cYf, phc, phf, mc, mf, dc, df, true, jf);
+  face_jat<P, C>(p, n.gXb, n.cYb, n.cYc, phb, phc, mb, mc, db, dc, true, jb);
+  face_jat<P, C>(p, n.gYc, n.cXc, n.cXr, phc, phr, mc, mr, dc, dr, y != p.yskip, jr);
+  face_jat<P, C>(p, n.gYl, n.cXl, n.cXc, phl, phc, ml, mc, dl, dc, (y - 1) != p.yskip, jl);
+  // face mobilities and mu gradients (calculate_gradients.h:148-165)
+  R Dc[C][C], Df[C][C], Db[C][C], Dr[C][C], Dl[C][C];
+  cell_D<P, C>(p, phc, Dc);
+  cell_D<P, C>(p, phf, Df);
+  cell_D<P, C>(p, phb, Db);
+  cell_D<P, C>(p, phr, Dr);
+  cell_D<P, C>(p, phl, Dl);
+  R Dmf[C][C], Dmb[C][C], Dmr[C][C], Dml[C][C], gmf[C], gmb[C], gmr[C], gml[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+#pragma unroll
+    for (int l = 0; l < C; l++) {
+      Dmf[k][l] = (Df[k][l] + Dc[k][l]) * R(0.5);
+      Dmb[k][l] = (Dc[k][l] + Db[k][l]) * R(0.5);
+      Dmr[k][l] = (Dr[k][l] + Dc[k][l]) * R(0.5);
+      Dml[k][l] = (Dc[k][l] + Dl[k][l]) * R(0.5);
+    }
+    gmf[k] = div_dx(mf[k] - mc[k], p);
+    gmb[k] = div_dx(mc[k] - mb[k], p);
+    gmr[k] = div_dy(mr[k] - mc[k], p);
+    gml[k] = div_dy(mc[k] - ml[k], p);
+  }
+  R pho[P], muo[C];
+  cell_update<P, C>(p, phc, mc, dc, jf, jb, jr, jl, Dmf, Dmb, Dmr, Dml, gmf, gmb, gmr, gml, pho, muo);
+#pragma unroll
+  for (int a = 0; a < P; a++) write_with_ghosts(p, 0, phi_o + (size_t)a * p.plane, x, y, pho[a]);
+#pragma unroll
+  for (int k = 0; k < C; k++) write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, muo[k]);
+}
+
+// Fill the ghost layers of a freshly loaded state: every interior cell copies
+// itself to its ghost images (apply_boundary_conditions(0) after loading).
+template <class R>
+__global__ void k_ghosts(const DevParams<R>* __restrict__ prm, R* __restrict__ phi, R* __restrict__ mu, int P, int C) {
+  const DevParams<R>& p = *prm;
+  const int y = 3 + blockIdx.x * BX + threadIdx.x;
+  const int x = 3 + blockIdx.y * BY + threadIdx.y;
+  if (x > p.mesh_x + 2 || y > p.mesh_y + 2) return;
+  for (int a = 0; a < P; a++) {
+    R* b = phi + (size_t)a * p.plane;
+    write_with_ghosts(p, 0, b, x, y, b[(size_t)x * p.pitch + y]);
+  }
+  for (int k = 0; k < C; k++) {
+    R* b = mu + (size_t)k * p.plane;
+    write_with_ghosts(p, 1, b, x, y, b[(size_t)x * p.pitch + y]);
+  }
+}
+
+// interior <-> padded copies (host layout [f][MX][MY] dense, R or double)
+template <class R, class S>
+__global__ void k_pack(const DevParams<R>* __restrict__ prm, const S* __restrict__ src, R* __restrict__ dst, int nf) {
+  const DevParams<R>& p = *prm;
+  const int y = blockIdx.x * BX + threadIdx.x;
+  const int x = blockIdx.y * BY + threadIdx.y;
+  if (x >= p.mesh_x || y >= p.mesh_y) return;
+  for (int f = 0; f < nf; f++)
+    dst[(size_t)f * p.plane + (size_t)(x + 3) * p.pitch + (y + 3)] = (R)src[((size_t)f * p.mesh_x + x) * p.mesh_y + y];
+}
+
+template <class R, class S>
+__global__ void k_unpack(const DevParams<R>* __restrict__ prm, const R* __restrict__ src, S* __restrict__ dst, int nf) {
+  const DevParams<R>& p = *prm;
+  const int y = blockIdx.x * BX + threadIdx.x;
+  const int x = blockIdx.y * BY + threadIdx.y;
+  if (x >= p.mesh_x || y >= p.mesh_y) return;
+  for (int f = 0; f < nf; f++)
+    dst[((size_t)f * p.mesh_x + x) * p.mesh_y + y] = (S)src[(size_t)f * p.plane + (size_t)(x + 3) * p.pitch + (y + 3)];
+}
+
+// Diagnostics: per-block partials (binary64) of max/min/sum of phi, max/min of mu,
+// solute sum_a h_a c_a(mu) with half weights on the first and last interior
+// node of each axis, and the count of non-finite values.
+// Partial layout per block: [phi_max P][phi_min P][phi_sum P][mu_max C][mu_min C][solute C][nonfinite 1]
+template <int P, int C, class R>
+__global__ void __launch_bounds__(256) k_stats(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                               const R* __restrict__ mu, double* __restrict__ part) {
+  const DevParams<R>& p = *prm;
+  constexpr int NV = 3 * P + 3 * C + 1;
+  double acc[NV];
+#pragma unroll
+  for (int a = 0; a < P; a++) { acc[a] = -INFINITY; acc[P + a] = INFINITY; acc[2 * P + a] = 0.0; }
+#pragma unroll
+  for (int k = 0; k < C; k++) { acc[3 * P + k] = -INFINITY; acc[3 * P + C + k] = INFINITY; acc[3 * P + 2 * C + k] = 0.0; }
+  acc[NV - 1] = 0.0;
+  const long ncell = (long)p.mesh_x * p.mesh_y;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += (long)gridDim.x * blockDim.x) {
+    int x = (int)(i / p.mesh_y) + 3, y = (int)(i % p.mesh_y) + 3;
+    double w = ((x == 3 || x == p.mesh_x + 2) ? 0.5 : 1.0) * ((y == 3 || y == p.mesh_y + 2) ? 0.5 : 1.0);
+    R ph[P], m[C];
+#pragma unroll
+    for (int a = 0; a < P; a++) {
+      ph[a] = phi[idx(p, a, x, y)];
+      double d = (double)ph[a];
+      if (!isfinite(d)) acc[NV - 1] += 1.0;
+      acc[a] = fmax(acc[a], d);
+      acc[P + a] = fmin(acc[P + a], d);
+      acc[2 * P + a] += d;
+    }
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      m[k] = mu[idx(p, k, x, y)];
+      double d = (double)m[k];
+      if (!isfinite(d)) acc[NV - 1] += 1.0;
+      acc[3 * P + k] = fmax(acc[3 * P + k], d);
+      acc[3 * P + C + k] = fmin(acc[3 * P + C + k], d);
+    }
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < P; a++) {
+        double ca = 0.0;
+#pragma unroll
+        for (int l = 0; l < C; l++) ca += (double)p.cmu[a][k][l] * ((double)m[l] - (double)p.Bsh[a][l]);
+        double pa = ph[a];
+        double s1 = 0.0;
+#pragma unroll
+        for (int b = 0; b < P; b++)
+#pragma unroll
+          for (int c = 0; c < P; c++)
+            if (b != a && c != a && b < c) s1 += (double)ph[b] * (double)ph[c];
+        double h = 3.0 * pa * pa - 2.0 * pa * pa * pa + 2.0 * pa * s1;
+        s += h * ca;
+      }
+      acc[3 * P + 2 * C + k] += w * s;
+    }
+  }
+  // block reduction through LDS
+  __shared__ double red[256];
+  for (int v = 0; v < NV; v++) {
+    red[threadIdx.x] = acc[v];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) {
+        double o = red[threadIdx.x + s], me = red[threadIdx.x];
+        bool ismax = (v < P) || (v >= 3 * P && v < 3 * P + C);
+        bool ismin = (v >= P && v < 2 * P) || (v >= 3 * P + C && v < 3 * P + 2 * C);
+        red[threadIdx.x] = ismax ? fmax(me, o) : ismin ? fmin(me, o) : me + o;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * NV + v] = red[0];
+    __syncthreads();
+  }
+}
+
+}  // namespace gpk

@@ -1,0 +1,321 @@
+// gp_step.hip -- C ABI of the MI355X grand-potential 2-D step (include/gpstep.h).
+//
+// Context creation copies the host-derived constant tables into a device
+// DevParams<R> block, allocates the padded field planes, and dispatches each
+// call to the kernels instantiated for the context's (P, C, real type).
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+
+#include "gp_ctx.hpp"
+#include "gp_kernels.hpp"
+#include "gpio.h"
+
+// (P, C) pairs with compiled kernels; other combinations are rejected with
+// GP_E_UNSUPPORTED.  Must match the GP_INSTANTIATE lines of gp_inst_p*.hip.
+#define GP_PC_LIST(X) \
+  X(2, 1) X(3, 1) X(4, 1) X(5, 1) X(6, 1) X(7, 1) X(8, 1) X(9, 1) \
+  X(2, 2) X(3, 2) X(4, 2) X(5, 2) X(6, 2) X(2, 3) X(3, 3) X(4, 3)
+
+static bool pc_supported(int P, int C) {
+#define GP_PC_TEST(P_, C_) if (P == P_ && C == C_) return true;
+  GP_PC_LIST(GP_PC_TEST)
+#undef GP_PC_TEST
+  return false;
+}
+
+template <class R>
+static void fill_devparams(const gp_params &h, int rx, int ry, int pitch, long plane, gpk::DevParams<R> &d) {
+  memset(&d, 0, sizeof(d));
+  const int P = h.nphases, C = h.ncomp, L = P - 1;
+  d.plane = plane;
+  d.rx = rx;
+  d.ry = ry;
+  d.pitch = pitch;
+  d.mesh_x = (int)h.mesh_x;
+  d.mesh_y = (int)h.mesh_y;
+  d.obstacle = h.obstacle;
+  d.anisotropy = h.anisotropy;
+  d.mu_solver = h.mu_solver;
+  d.yskip = (int)h.mesh_y - 2;
+  for (int f = 0; f < 2; f++) {
+    // slots 0,1,2 = ghosts 0,1,2 (faces X- / Y-), slots 3,4,5 = ghosts n-3, n-2, n-1 (X+ / Y+)
+    d.gsrc[f][0][0] = (int)h.bc_proxy[f][GP_FACE_XMIN][2];
+    d.gsrc[f][0][1] = (int)h.bc_proxy[f][GP_FACE_XMIN][1];
+    d.gsrc[f][0][2] = (int)h.bc_proxy[f][GP_FACE_XMIN][0];
+    d.gsrc[f][0][3] = (int)h.bc_proxy[f][GP_FACE_XMAX][0];
+    d.gsrc[f][0][4] = (int)h.bc_proxy[f][GP_FACE_XMAX][1];
+    d.gsrc[f][0][5] = (int)h.bc_proxy[f][GP_FACE_XMAX][2];
+    d.gsrc[f][1][0] = (int)h.bc_proxy[f][GP_FACE_YMIN][2];
+    d.gsrc[f][1][1] = (int)h.bc_proxy[f][GP_FACE_YMIN][1];
+    d.gsrc[f][1][2] = (int)h.bc_proxy[f][GP_FACE_YMIN][0];
+    d.gsrc[f][1][3] = (int)h.bc_proxy[f][GP_FACE_YMAX][0];
+    d.gsrc[f][1][4] = (int)h.bc_proxy[f][GP_FACE_YMAX][1];
+    d.gsrc[f][1][5] = (int)h.bc_proxy[f][GP_FACE_YMAX][2];
+  }
+  // scalar constants, each formed in binary64 as the reference forms it
+  d.dx = (R)h.dx;
+  d.dy = (R)h.dy;
+  d.dt = (R)h.dt;
+  d.two_dy = (R)(2.0 * h.dy);
+  d.inv_dx = (R)(1.0 / h.dx);
+  d.inv_dy = (R)(1.0 / h.dy);
+  d.inv_two_dy = (R)(1.0 / (2.0 * h.dy));
+  d.epsilon = (R)h.epsilon;
+  d.inv_eps = (R)(1.0 / h.epsilon);
+  d.tau = (R)h.tau;
+  d.w_pref = (R)(h.obstacle ? 16.0 / (M_PI * M_PI) : 9.0);
+  d.jscale = (R)(h.obstacle ? (-0.25 * M_PI * h.epsilon) : (-1.0 * h.epsilon));
+  d.well_at = (R)(0.5 / sqrt(2.0));
+  for (int a = 0; a < P; a++) {
+    for (int b = 0; b < P; b++) {
+      d.gamma[a][b] = (R)h.gamma[a][b];
+      d.tau_ab[a][b] = (R)h.tau_ab[a][b];
+      d.dab[a][b] = (R)h.dab[a][b];
+      for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+          d.rot[a][b][i][j] = (R)h.rot[a][b][i][j];
+          d.rot_inv[a][b][i][j] = (R)h.rot_inv[a][b][i][j];
+        }
+      for (int c = 0; c < P; c++) d.gamma_abc[a][b][c] = (R)h.gamma_abc[a][b][c];
+    }
+    for (int k = 0; k < C; k++) {
+      d.at_coef[a][k] = (R)(a < L ? (1.0 - h.diffusivity[a][k][k] / h.diffusivity[L][k][k]) : 0.0);
+      d.Bsh[a][k] = (R)(h.Beq[a][k] + h.dBbdT[a][k] * (h.T - h.Teq));
+      d.B[a][k] = (R)h.B[a][k];
+      for (int l = 0; l < C; l++) {
+        d.A[a][k][l] = (R)h.A[a][k][l];
+        d.cmu[a][k][l] = (R)h.cmu[a][k][l];
+        for (int m = 0; m < C; m++) d.dcm[a][k][l][m] = (R)(h.diffusivity[a][k][m] * h.cmu[a][m][l]);
+      }
+    }
+    d.Cf[a] = (R)h.Cf[a];
+  }
+}
+
+template <class R, class S>
+static int load_state(gp_ctx *c, const S *dphi_src, const S *dmu_src) {
+  const gpk::DevParams<R> *prm = (const gpk::DevParams<R> *)c->dprm;
+  dim3 blk(gpk::BX, gpk::BY), g((c->hp.mesh_y + gpk::BX - 1) / gpk::BX, (c->hp.mesh_x + gpk::BY - 1) / gpk::BY);
+  dim3 gi((c->hp.mesh_y + gpk::BX - 1) / gpk::BX, (c->hp.mesh_x + gpk::BY - 1) / gpk::BY);
+  hipLaunchKernelGGL((gpk::k_pack<R, S>), g, blk, 0, c->stream, prm, dphi_src, (R *)c->phi[c->cur], c->P);
+  hipLaunchKernelGGL((gpk::k_pack<R, S>), g, blk, 0, c->stream, prm, dmu_src, (R *)c->mu[c->cur], c->C);
+  hipLaunchKernelGGL((gpk::k_ghosts<R>), gi, blk, 0, c->stream, prm, (R *)c->phi[c->cur], (R *)c->mu[c->cur], c->P, c->C);
+  if (c->mulo[c->cur]) GP_HIP_CHECK(hipMemsetAsync(c->mulo[c->cur], 0, (size_t)c->plane * c->esize * c->C, c->stream));
+  GP_HIP_CHECK(hipGetLastError());
+  return GP_OK;
+}
+
+template <class R, class S>
+static int store_state(gp_ctx *c, S *dphi_dst, S *dmu_dst) {
+  const gpk::DevParams<R> *prm = (const gpk::DevParams<R> *)c->dprm;
+  dim3 blk(gpk::BX, gpk::BY), g((c->hp.mesh_y + gpk::BX - 1) / gpk::BX, (c->hp.mesh_x + gpk::BY - 1) / gpk::BY);
+  hipLaunchKernelGGL((gpk::k_unpack<R, S>), g, blk, 0, c->stream, prm, (const R *)c->phi[c->cur], dphi_dst, c->P);
+  hipLaunchKernelGGL((gpk::k_unpack<R, S>), g, blk, 0, c->stream, prm, (const R *)c->mu[c->cur], dmu_dst, c->C);
+  GP_HIP_CHECK(hipGetLastError());
+  return GP_OK;
+}
+
+extern "C" {
+
+const char *gp_strerror(int code) {
+  switch (code) {
+    case GP_OK: return "ok";
+    case GP_E_ARG: return "invalid argument or parameters";
+    case GP_E_UNSUPPORTED: return "valid input not supported by this build";
+    case GP_E_HIP: return "HIP runtime error";
+    case GP_E_NOMEM: return "allocation failed";
+    case GP_E_NONFINITE: return "non-finite values in the fields";
+    case GP_E_IO: return "file not found or unreadable";
+    case GP_E_NODEVICE: return "no HIP device";
+    default: return "unknown error";
+  }
+}
+
+static void free_ctx(gp_ctx *c) {
+  if (!c) return;
+  for (int i = 0; i < 2; i++) {
+    if (c->phi[i]) (void)hipFree(c->phi[i]);
+    if (c->mu[i]) (void)hipFree(c->mu[i]);
+    if (c->mulo[i]) (void)hipFree(c->mulo[i]);
+  }
+  if (c->dphi) (void)hipFree(c->dphi);
+  if (c->dprm) (void)hipFree(c->dprm);
+  if (c->dpart) (void)hipFree(c->dpart);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  delete c;
+}
+
+int gp_create(const gp_params *p, int dtype, gp_ctx **out) {
+  if (!p || !out || (dtype != GP_F64 && dtype != GP_F32)) return GP_E_ARG;
+  *out = nullptr;
+  char err[200];
+  int rc = gp_check_supported(p, err, sizeof err);
+  if (rc) return rc;
+  if (!pc_supported(p->nphases, p->ncomp)) return GP_E_UNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return GP_E_NODEVICE;
+  gp_ctx *c = new (std::nothrow) gp_ctx();
+  if (!c) return GP_E_NOMEM;
+  c->hp = *p;
+  c->dtype = dtype;
+  c->P = p->nphases;
+  c->C = p->ncomp;
+  c->rx = (int)p->mesh_x + 6;
+  c->ry = (int)p->mesh_y + 6;
+  c->esize = dtype == GP_F64 ? 8 : 4;
+  int align = (int)(256 / c->esize);
+  c->pitch = (c->ry + align - 1) / align * align;
+  c->plane = (long)c->rx * c->pitch;
+  (void)hipGetDevice(&c->device);
+  size_t pb = (size_t)c->plane * c->esize;
+  bool ok = true;
+  for (int i = 0; i < 2 && ok; i++) {
+    ok = hipMalloc(&c->phi[i], pb * c->P) == hipSuccess && hipMalloc(&c->mu[i], pb * c->C) == hipSuccess &&
+         (dtype == GP_F64 || hipMalloc(&c->mulo[i], pb * c->C) == hipSuccess);
+    if (ok) ok = hipMemset(c->phi[i], 0, pb * c->P) == hipSuccess && hipMemset(c->mu[i], 0, pb * c->C) == hipSuccess &&
+                 (dtype == GP_F64 || hipMemset(c->mulo[i], 0, pb * c->C) == hipSuccess);
+  }
+  ok = ok && hipMalloc(&c->dphi, pb * c->P) == hipSuccess && hipMemset(c->dphi, 0, pb * c->P) == hipSuccess;
+  long ncell = p->mesh_x * p->mesh_y;
+  c->nstat_blocks = (int)std::min<long>(1024, (ncell + 255) / 256);
+  int nv = 3 * c->P + 3 * c->C + 1;
+  ok = ok && hipMalloc((void **)&c->dpart, sizeof(double) * c->nstat_blocks * nv) == hipSuccess;
+  if (dtype == GP_F64) {
+    gpk::DevParams<double> *h = new (std::nothrow) gpk::DevParams<double>();
+    ok = ok && h;
+    if (ok) {
+      fill_devparams(*p, c->rx, c->ry, c->pitch, c->plane, *h);
+      ok = hipMalloc(&c->dprm, sizeof(*h)) == hipSuccess && hipMemcpy(c->dprm, h, sizeof(*h), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    delete h;
+  } else {
+    gpk::DevParams<float> *h = new (std::nothrow) gpk::DevParams<float>();
+    ok = ok && h;
+    if (ok) {
+      fill_devparams(*p, c->rx, c->ry, c->pitch, c->plane, *h);
+      ok = hipMalloc(&c->dprm, sizeof(*h)) == hipSuccess && hipMemcpy(c->dprm, h, sizeof(*h), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    delete h;
+  }
+  ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
+  if (!ok) {
+    free_ctx(c);
+    return GP_E_NOMEM;
+  }
+  c->stream = nullptr;
+  c->cur = 0;
+  c->path = 1;
+  *out = c;
+  return GP_OK;
+}
+
+void gp_destroy(gp_ctx *c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->stream);
+  free_ctx(c);
+}
+
+int gp_set_stream(gp_ctx *c, void *s) {
+  if (!c) return GP_E_ARG;
+  c->stream = (hipStream_t)s;
+  return GP_OK;
+}
+
+int gp_set_path(gp_ctx *c, int path) {
+  if (!c || path < 0 || path > 1) return GP_E_ARG;
+  c->path = path;
+  return GP_OK;
+}
+
+int gp_set_state(gp_ctx *c, const double *phi, const double *mu) {
+  if (!c || !phi || !mu) return GP_E_ARG;
+  size_t n = (size_t)c->hp.mesh_x * c->hp.mesh_y;
+  double *d = nullptr;
+  if (hipMalloc((void **)&d, sizeof(double) * n * (c->P + c->C)) != hipSuccess) return GP_E_NOMEM;
+  int rc = GP_OK;
+  if (hipMemcpyAsync(d, phi, sizeof(double) * n * c->P, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(d + n * c->P, mu, sizeof(double) * n * c->C, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = GP_E_HIP;
+  if (!rc) rc = c->dtype == GP_F64 ? load_state<double, double>(c, d, d + n * c->P) : load_state<float, double>(c, d, d + n * c->P);
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = GP_E_HIP;
+  (void)hipFree(d);
+  return rc;
+}
+
+int gp_get_state(gp_ctx *c, double *phi, double *mu) {
+  if (!c || !phi || !mu) return GP_E_ARG;
+  size_t n = (size_t)c->hp.mesh_x * c->hp.mesh_y;
+  double *d = nullptr;
+  if (hipMalloc((void **)&d, sizeof(double) * n * (c->P + c->C)) != hipSuccess) return GP_E_NOMEM;
+  int rc = c->dtype == GP_F64 ? store_state<double, double>(c, d, d + n * c->P) : store_state<float, double>(c, d, d + n * c->P);
+  if (!rc && (hipMemcpyAsync(phi, d, sizeof(double) * n * c->P, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipMemcpyAsync(mu, d + n * c->P, sizeof(double) * n * c->C, hipMemcpyDeviceToHost, c->stream) != hipSuccess))
+    rc = GP_E_HIP;
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = GP_E_HIP;
+  (void)hipFree(d);
+  return rc;
+}
+
+int gp_set_state_device(gp_ctx *c, const void *phi, const void *mu) {
+  if (!c || !phi || !mu) return GP_E_ARG;
+  return c->dtype == GP_F64 ? load_state<double, double>(c, (const double *)phi, (const double *)mu)
+                            : load_state<float, float>(c, (const float *)phi, (const float *)mu);
+}
+
+int gp_get_state_device(gp_ctx *c, void *phi, void *mu) {
+  if (!c || !phi || !mu) return GP_E_ARG;
+  return c->dtype == GP_F64 ? store_state<double, double>(c, (double *)phi, (double *)mu)
+                            : store_state<float, float>(c, (float *)phi, (float *)mu);
+}
+
+int gp_step(gp_ctx *c, long nsteps) {
+  if (!c || nsteps < 0) return GP_E_ARG;
+  GP_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+  int rc = GP_E_UNSUPPORTED;
+#define GP_DISPATCH_STEP(P_, C_)                                                                   \
+  if (c->P == P_ && c->C == C_)                                                                    \
+    rc = c->dtype == GP_F64 ? gpk::run_steps<P_, C_, double>(c, nsteps) : gpk::run_steps<P_, C_, float>(c, nsteps);
+  GP_PC_LIST(GP_DISPATCH_STEP)
+#undef GP_DISPATCH_STEP
+  if (rc) return rc;
+  GP_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+  GP_HIP_CHECK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  GP_HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  return GP_OK;
+}
+
+int gp_last_step_ms(gp_ctx *c, double *ms) {
+  if (!c || !ms) return GP_E_ARG;
+  *ms = c->last_ms;
+  return GP_OK;
+}
+
+int gp_stats_get(gp_ctx *c, gp_stats *st) {
+  if (!c || !st) return GP_E_ARG;
+  memset(st, 0, sizeof(*st));
+  int rc = GP_E_UNSUPPORTED;
+#define GP_DISPATCH_STATS(P_, C_)                                                                 \
+  if (c->P == P_ && c->C == C_)                                                                   \
+    rc = c->dtype == GP_F64 ? gpk::run_stats<P_, C_, double>(c, st) : gpk::run_stats<P_, C_, float>(c, st);
+  GP_PC_LIST(GP_DISPATCH_STATS)
+#undef GP_DISPATCH_STATS
+  if (rc) return rc;
+  return st->nonfinite > 0 ? GP_E_NONFINITE : GP_OK;
+}
+
+// sizes of the ABI structures, so bindings can check their layouts
+int gp_abi_sizes(size_t *params, size_t *stats, size_t *input) {
+  if (params) *params = sizeof(gp_params);
+  if (stats) *stats = sizeof(gp_stats);
+  if (input) *input = sizeof(gp_input);
+  return GP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+"""Python binding of libgpstep.so (the C ABI in include/gpstep.h, include/gpio.h).
+
+The binding mirrors the reference solver's interface for this path: an Input.in
+is read into parameters (reading_input_parameters + init_propertymatrices), an
+initial state comes from Filling.in or a VTK restart file, and `step(n)` runs n
+iterations of the time-loop body of microsim_gp.c:103-106.  All compute runs in
+the HIP library; there is no Python or CPU fallback, and a missing library is an
+error.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+from ._abi import GP_F32, GP_F64, GpInput, GpParams, GpStats
+
+_LIBPATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgpstep.so")
+_lib = None
+
+
+class GPError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = lib().gp_strerror(code).decode() if _lib is not None else str(code)
+        super().__init__(f"{what}: {msg} ({code})" if what else f"{msg} ({code})")
+
+
+def lib():
+    """The loaded libgpstep.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIBPATH):
+            raise RuntimeError(f"{_LIBPATH} is missing: build it with `python -m eshwar_dipan_amd.build`")
+        L = ct.CDLL(_LIBPATH)
+        vp, i, l, d = ct.c_void_p, ct.c_int, ct.c_long, ct.c_double
+        L.gp_strerror.restype = ct.c_char_p
+        L.gp_strerror.argtypes = [i]
+        for name, args in [
+            ("gp_create", [ct.POINTER(GpParams), i, ct.POINTER(vp)]),
+            ("gp_set_state", [vp, vp, vp]), ("gp_get_state", [vp, vp, vp]),
+            ("gp_set_state_device", [vp, vp, vp]), ("gp_get_state_device", [vp, vp, vp]),
+            ("gp_step", [vp, l]), ("gp_stats_get", [vp, ct.POINTER(GpStats)]),
+            ("gp_set_stream", [vp, vp]), ("gp_last_step_ms", [vp, ct.POINTER(d)]),
+            ("gp_set_path", [vp, i]),
+            ("gp_read_input", [ct.c_char_p, ct.POINTER(GpInput), i, ct.c_char_p, ct.c_size_t]),
+            ("gp_derive", [ct.POINTER(GpParams)]),
+            ("gp_check_supported", [ct.POINTER(GpParams), ct.c_char_p, ct.c_size_t]),
+            ("gp_fill_domain", [ct.POINTER(GpInput), ct.c_char_p, vp, vp, i, ct.c_char_p, ct.c_size_t]),
+            ("gp_vtk_write", [ct.c_char_p, ct.POINTER(GpInput), vp, vp, i]),
+            ("gp_vtk_read", [ct.c_char_p, ct.POINTER(GpInput), vp, vp, i]),
+            ("gp_matinv", [vp, vp, i]),
+            ("gp_abi_sizes", [ct.POINTER(ct.c_size_t)] * 3),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = i
+        L.gp_destroy.argtypes = [vp]
+        L.gp_destroy.restype = None
+        L.gp_mu_of_c.argtypes = [ct.POINTER(GpParams), vp, d, i, i]
+        L.gp_mu_of_c.restype = d
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+def _check(rc, what=""):
+    if rc != 0:
+        raise GPError(rc, what)
+
+
+def read_input(path, verbose=False):
+    """Parse an Input.in and derive the constant tables; returns a GpInput."""
+    inp = GpInput()
+    err = ct.create_string_buffer(256)
+    rc = lib().gp_read_input(os.fsencode(path), ct.byref(inp), int(verbose), err, 256)
+    if rc != 0:
+        raise GPError(rc, f"{path}: {err.value.decode()}")
+    _check(lib().gp_derive(ct.byref(inp.p)), "gp_derive")
+    return inp
+
+
+def read_input_text(text, workdir):
+    path = os.path.join(workdir, "Input.in")
+    with open(path, "w") as f:
+        f.write(text)
+    return read_input(path)
+
+
+def fill_domain(inp, filling_path):
+    p = inp.p
+    phi = np.zeros((p.nphases, p.mesh_x, p.mesh_y))
+    mu = np.zeros((p.ncomp, p.mesh_x, p.mesh_y))
+    err = ct.create_string_buffer(256)
+    rc = lib().gp_fill_domain(ct.byref(inp), os.fsencode(filling_path), _ptr(phi), _ptr(mu), 0, err, 256)
+    if rc != 0:
+        raise GPError(rc, err.value.decode())
+    return phi, mu
+
+
+def vtk_read(path, inp, binary=True):
+    p = inp.p
+    phi = np.zeros((p.nphases, p.mesh_x, p.mesh_y))
+    mu = np.zeros((p.ncomp, p.mesh_x, p.mesh_y))
+    _check(lib().gp_vtk_read(os.fsencode(path), ct.byref(inp), _ptr(phi), _ptr(mu), int(binary)), path)
+    return phi, mu
+
+
+def vtk_write(path, inp, phi, mu, binary=True):
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    _check(lib().gp_vtk_write(os.fsencode(path), ct.byref(inp), _ptr(phi), _ptr(mu), int(binary)), path)
+
+
+class Solver:
+    """One grand-potential field pair on the GPU (a gp_ctx).
+
+    dtype: "f64" (binary64, bit-compatible with the reference) or "f32"
+    (binary32 bulk stencil, compensated mu, binary64 reductions).
+    """
+
+    def __init__(self, params, dtype="f64", path=None):
+        if isinstance(params, GpInput):
+            params = params.p
+        self.params = params
+        self.dtype = dtype
+        h = ct.c_void_p()
+        _check(lib().gp_create(ct.byref(params), GP_F64 if dtype == "f64" else GP_F32, ct.byref(h)), "gp_create")
+        self._h = h
+        if path is not None:
+            _check(lib().gp_set_path(h, int(path)), "gp_set_path")
+
+    def close(self):
+        if self._h:
+            lib().gp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        p = self.params
+        return (p.nphases, p.mesh_x, p.mesh_y), (p.ncomp, p.mesh_x, p.mesh_y)
+
+    def set_state(self, phi, mu):
+        sp, sm = self.shape
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        if phi.shape != sp or mu.shape != sm:
+            raise ValueError(f"state shapes {phi.shape}, {mu.shape} != {sp}, {sm}")
+        _check(lib().gp_set_state(self._h, _ptr(phi), _ptr(mu)), "gp_set_state")
+
+    def get_state(self):
+        sp, sm = self.shape
+        phi = np.empty(sp)
+        mu = np.empty(sm)
+        _check(lib().gp_get_state(self._h, _ptr(phi), _ptr(mu)), "gp_get_state")
+        return phi, mu
+
+    def set_state_device(self, phi_ptr, mu_ptr):
+        _check(lib().gp_set_state_device(self._h, ct.c_void_p(phi_ptr), ct.c_void_p(mu_ptr)), "gp_set_state_device")
+
+    def get_state_device(self, phi_ptr, mu_ptr):
+        _check(lib().gp_get_state_device(self._h, ct.c_void_p(phi_ptr), ct.c_void_p(mu_ptr)), "gp_get_state_device")
+
+    def set_stream(self, stream_ptr):
+        _check(lib().gp_set_stream(self._h, ct.c_void_p(stream_ptr)), "gp_set_stream")
+
+    def step(self, n=1):
+        _check(lib().gp_step(self._h, int(n)), "gp_step")
+
+    def last_step_ms(self):
+        v = ct.c_double()
+        _check(lib().gp_last_step_ms(self._h, ct.byref(v)))
+        return v.value
+
+    def stats(self, allow_nonfinite=False):
+        st = GpStats()
+        rc = lib().gp_stats_get(self._h, ct.byref(st))
+        if rc != 0 and not (allow_nonfinite and rc == -5):
+            raise GPError(rc, "gp_stats_get")
+        P, C = self.params.nphases, self.params.ncomp
+        return {
+            "phi_max": list(st.phi_max[:P]), "phi_min": list(st.phi_min[:P]),
+            "phi_sum": list(st.phi_sum[:P]), "mu_max": list(st.mu_max[:C]),
+            "mu_min": list(st.mu_min[:C]), "solute": list(st.solute[:C]),
+            "nonfinite": st.nonfinite,
+        }

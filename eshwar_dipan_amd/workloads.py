@@ -1,0 +1,115 @@
+"""Named synthetic workloads of the grand-potential step.
+
+Each builder returns an Input.in-derived parameter set (through the library's own
+parser and constant derivation) and a seeded initial state.  The five workloads
+follow the benchmark configurations of this path:
+
+  c1  2 phases, binary, 256^2, isotropic, one seed r=10, Neumann, 1000 steps
+  c2  3 solids + liquid, ternary, 1024^2, 4-fold anisotropy 0.02, 32 seeds r in
+      [6,12] with random labels, mu + N(0, 1e-3), periodic, 10000 steps
+  c3  8 solids + liquid, binary, 4096^2, Voronoi grains from 200 sites, 20000 steps
+  c4  ensemble of 2-phase binary instances of uneven sizes (batched path)
+  c5  2-phase binary dendrite, 512^2, 4-fold anisotropy 0.02, 2000 steps
+"""
+import ctypes as ct
+import os
+import tempfile
+
+import numpy as np
+
+from . import gpstep, synth
+
+
+def params(P, C, mesh, aniso=0.0, W=1, bc=(1, 1, 1, 1), solver=None, gabc=None, rot=None,
+           dt=None, eps=None, D_sol=0.0, A_off=0.0, T=None, extra=()):
+    """GpInput for a synthetic configuration, written as Input.in text and read
+    back through gp_read_input + gp_derive."""
+    keys = synth.base_params(P, C, mesh)
+    npair = P * (P - 1) // 2
+    if aniso:
+        keys["Function_anisotropy"] = "1"
+        keys["dab"] = "{" + ",".join([str(aniso)] * npair) + "}"
+    keys["Function_W"] = str(W)
+    if gabc is not None and P >= 3:
+        keys["Gamma_abc"] = "{" + ",".join([str(gabc)] * (P * (P - 1) * (P - 2) // 6)) + "}"
+    if dt is not None:
+        keys["DELTA_t"] = str(dt)
+    if eps is not None:
+        keys["epsilon"] = str(eps)
+    if T is not None:
+        keys["T"] = str(T)
+    solver = solver or ("BINARY" if C == 1 else "TERNARY" if C == 2 else "GENERAL")
+    lines = synth.thermo_lines(P, C, solver=solver, D_sol=D_sol, A_off=A_off)
+    lines += synth.bc_lines(bc, bc)
+    if aniso:
+        lines += synth.rotation_lines(P, rot)
+    lines += list(extra)
+    d = tempfile.mkdtemp(prefix="gpw_")
+    try:
+        return gpstep.read_input_text(synth.infile_text(keys, lines), d)
+    finally:
+        for f in os.listdir(d):
+            os.remove(os.path.join(d, f))
+        os.rmdir(d)
+
+
+def liquid_mu(inp):
+    """Mu of the liquid at its filling composition and Teq, per component."""
+    p = inp.p
+    P, C = p.nphases, p.ncomp
+    cc = np.zeros(4)
+    cc[:C] = [p.cfill[P - 1][P - 1][j] for j in range(C)]
+    return [gpstep.lib().gp_mu_of_c(ct.byref(p), cc.ctypes.data_as(ct.c_void_p), p.Teq, P - 1, k)
+            for k in range(C)]
+
+
+def seeded_state(inp, nseeds, rmin, rmax, seed, diffuse=0.0, noise=0.0):
+    p = inp.p
+    P, C, MX, MY = p.nphases, p.ncomp, p.mesh_x, p.mesh_y
+    phi = synth.seeds(MX, MY, P, nseeds, rmin, rmax, seed, diffuse=diffuse, periodic=p.bc_type[0][0] == 3)
+    ml = liquid_mu(inp)
+    mu = synth.mu_fill(MX, MY, [0.0] * C, lambda cl, k: ml[k], noise=noise, seed=seed + 1)
+    return phi, mu
+
+
+def voronoi_state(inp, npoints, seed, liquid_fraction=0.0):
+    p = inp.p
+    phi = synth.voronoi(p.mesh_x, p.mesh_y, p.nphases, npoints, seed, liquid_fraction=liquid_fraction)
+    ml = liquid_mu(inp)
+    mu = synth.mu_fill(p.mesh_x, p.mesh_y, [0.0] * p.ncomp, lambda cl, k: ml[k])
+    return phi, mu
+
+
+def c1(mesh=256):
+    inp = params(2, 1, (mesh, mesh))
+    phi, mu = seeded_state(inp, 0, 1, 1, seed=1)
+    X, Y = np.meshgrid(np.arange(mesh), np.arange(mesh), indexing="ij")
+    disk = (X - mesh // 2) ** 2 + (Y - mesh // 2) ** 2 <= 10 ** 2
+    phi[0][disk], phi[1][disk] = 1.0, 0.0
+    return inp, phi, mu, 1000, f"2-phase binary {mesh}x{mesh} isotropic, one seed r=10, Neumann"
+
+
+def c2(mesh=1024):
+    inp = params(4, 2, (mesh, mesh), aniso=0.02, bc=(3, 3, 3, 3), gabc=1.0)
+    phi, mu = seeded_state(inp, 32, 6, 12, seed=2, noise=1e-3)
+    return inp, phi, mu, 10000, (f"3 solids + liquid, ternary, {mesh}x{mesh}, 4-fold anisotropy 0.02, "
+                                 "32 seeds r 6-12, mu + N(0,1e-3), periodic")
+
+
+def c3(mesh=4096):
+    inp = params(9, 1, (mesh, mesh), gabc=0.5)
+    phi, mu = voronoi_state(inp, 200, seed=3)
+    return inp, phi, mu, 20000, f"8 solids + liquid, binary, {mesh}x{mesh}, Voronoi 200 sites"
+
+
+def c5(mesh=512):
+    inp = params(2, 1, (mesh, mesh), aniso=0.02)
+    phi, mu = seeded_state(inp, 0, 1, 1, seed=5)
+    X, Y = np.meshgrid(np.arange(mesh), np.arange(mesh), indexing="ij")
+    r = np.sqrt((X - mesh / 2) ** 2 + (Y - mesh / 2) ** 2)
+    phi[0] = 0.5 * (1 - np.tanh((r - 8.0) / 1.5))
+    phi[1] = 1.0 - phi[0]
+    return inp, phi, mu, 2000, f"2-phase binary dendrite {mesh}x{mesh}, 4-fold anisotropy 0.02"
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c5": c5}

@@ -82,6 +82,10 @@ int gp_vtk_read(const char *path, const gp_input *in, double *phi, double *mu,
    its in-place rewrite of the input matrix; a and inv are n*n row-major, n <= 4. */
 void gp_matinv(double *a, double *inv, int n);
 
+/* sizeof(gp_params), sizeof(gp_stats), sizeof(gp_input) as compiled into the
+   library, so that bindings can check their structure layouts. */
+int gp_abi_sizes(size_t *params, size_t *stats, size_t *input);
+
 /* Chemical potential Mu_a(c) at temperature T (functionF_01.h:19-29). */
 double gp_mu_of_c(const gp_params *p, const double *c, double T, int a, int i);
 

@@ -1,0 +1,53 @@
+"""The C-ABI library loads (no GPU needed), exports every declared entry point and
+agrees with the Python structure mirrors."""
+import ctypes as ct
+import glob
+import os
+import re
+
+from eshwar_dipan_amd import gpstep
+from eshwar_dipan_amd._abi import GpInput, GpParams, GpStats
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?(?:int|void|double|char|float)\s*\*?\s*(gp_\w+)\s*\(", txt, re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = gpstep.lib()
+    names = declared_functions()
+    assert len(names) >= 20, names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match():
+    a, b, c = ct.c_size_t(), ct.c_size_t(), ct.c_size_t()
+    assert gpstep.lib().gp_abi_sizes(ct.byref(a), ct.byref(b), ct.byref(c)) == 0
+    assert a.value == ct.sizeof(GpParams)
+    assert b.value == ct.sizeof(GpStats)
+    assert c.value == ct.sizeof(GpInput)
+
+
+def test_strerror_names_codes():
+    L = gpstep.lib()
+    for code in range(0, -8, -1):
+        assert L.gp_strerror(code)
+    assert L.gp_strerror(-2) == b"valid input not supported by this build"
+
+
+def test_create_rejects_bad_arguments():
+    L = gpstep.lib()
+    h = ct.c_void_p()
+    p = GpParams()
+    # all-zero parameters are invalid before any device is touched
+    assert L.gp_create(ct.byref(p), 0, ct.byref(h)) != 0
+    assert L.gp_create(None, 0, ct.byref(h)) == -1
