@@ -629,7 +629,7 @@ __device__ __forceinline__ void cell_update(const DevParams<R>& p, const R* ph, 
                                             const R (&Dmf)[C][C], const R (&Dmb)[C][C],
                                             const R (&Dmr)[C][C], const R (&Dml)[C][C],
                                             const R* gmf, const R* gmb, const R* gmr, const R* gml,
-                                            R* ph_o, R* mu_o) {
+                                            R* ph_o, R* mu_o, R* dmu) {
   R divflux[C], divjat[C];
 #pragma unroll
   for (int k = 0; k < C; k++) {
@@ -647,14 +647,23 @@ __device__ __forceinline__ void cell_update(const DevParams<R>& p, const R* ph, 
   for (int a = P - 1; a >= 0; a--)
     if (ph[a] == R(1.0)) bulk = a;
   if (bulk >= 0) {
-    R cn[C];
+    R cn[C], dcb[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) dcb[k] = p.dt * (divflux[k]) - divjat[k];
 #pragma unroll
     for (int a = 0; a < P; a++)
       if (a == bulk) {
 #pragma unroll
         for (int k = 0; k < C; k++) cn[k] = c_mu<C>(p, mu, a, k) + p.dt * (divflux[k]) - divjat[k];
 #pragma unroll
-        for (int k = 0; k < C; k++) mu_o[k] = Mu_of_c<C>(p, cn, a, k);
+        for (int k = 0; k < C; k++) {
+          mu_o[k] = Mu_of_c<C>(p, cn, a, k);
+          // the same update as an increment of the linear map Mu(c): dMu/dc . delta_c
+          R d = R(0);
+#pragma unroll
+          for (int j = 0; j < C; j++) d += (j == k ? R(2.0) * p.A[a][k][k] : p.A[a][k][j]) * dcb[j];
+          dmu[k] = d;
+        }
       }
   } else {
     R sum[C], dc[C], dcdmu[C][C], h[P];
@@ -680,10 +689,14 @@ __device__ __forceinline__ void cell_update(const DevParams<R>& p, const R* ph, 
       }
     }
 #pragma unroll
-    for (int k = 0; k < C; k++) mu_o[k] = mu[k];
+    for (int k = 0; k < C; k++) {
+      mu_o[k] = mu[k];
+      dmu[k] = R(0);
+    }
     if (p.mu_solver == GP_MU_BINARY) {
       R inv = R(1.0) / dcdmu[0][0];
-      mu_o[0] = mu[0] + dc[0] * inv;
+      dmu[0] = dc[0] * inv;
+      mu_o[0] = mu[0] + dmu[0];
     } else if (C == 2 && p.mu_solver == GP_MU_TERNARY) {
       R DET = dcdmu[0][0] * dcdmu[C - 1][C - 1] - dcdmu[0][C - 1] * dcdmu[C - 1][0];
       R i00 = dcdmu[C - 1][C - 1] / DET, i11 = dcdmu[0][0] / DET;
@@ -693,13 +706,16 @@ __device__ __forceinline__ void cell_update(const DevParams<R>& p, const R* ph, 
       d0 += i01 * dc[C - 1];
       d1 += i10 * dc[0];
       d1 += i11 * dc[C - 1];
+      dmu[0] = d0;
+      dmu[C - 1] = d1;
       mu_o[0] = d0 + mu[0];
       mu_o[C - 1] = d1 + mu[C - 1];
     } else if (p.mu_solver == GP_MU_DILUTE) {
 #pragma unroll
       for (int k = 0; k < C; k++) {
         R inv = R(1.0) / dcdmu[k][k];
-        mu_o[k] = mu[k] + dc[k] * inv;
+        dmu[k] = dc[k] * inv;
+        mu_o[k] = mu[k] + dmu[k];
       }
     } else {
       R inv[C][C];
@@ -709,6 +725,7 @@ __device__ __forceinline__ void cell_update(const DevParams<R>& p, const R* ph, 
         R s = R(0);
 #pragma unroll
         for (int l = 0; l < C; l++) s = s + inv[k][l] * dc[l];
+        dmu[k] = s;
         mu_o[k] = s + mu[k];
       }
     }

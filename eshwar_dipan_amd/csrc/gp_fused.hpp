@@ -1,0 +1,167 @@
+// gp_fused.hpp -- one-launch step: LDS-staged tile of TX x TY interior cells.
+//
+//   1. phi of the tile plus a 2-cell halo and mu plus a 1-cell halo are read
+//      from HBM once into LDS (every field value of the step is read once per
+//      tile; halo re-reads hit L2);
+//   2. delta-phi is computed for the tile plus a 1-cell ring into LDS (the ring
+//      is the neighbours' delta-phi that the antitrapping flux needs);
+//   3. mu and phi of the tile are updated and written once, together with the
+//      ghost cells that copy them.
+// The arithmetic per value is the cell/face functions of gp_cell.hpp, so the
+// binary64 instantiation equals the two-pass kernels bit for bit.
+//
+// binary32: mu carries a low-order word (mulo); the interface and bulk mu
+// increments are added with Kahan compensation, and the bulk update is applied as
+// the increment dMu/dc . delta_c of the linear F_01 map instead of Mu(c(mu)+dc),
+// which is the same value in exact arithmetic without the round trip through c.
+#pragma once
+
+#include "gp_kernels.hpp"
+
+namespace gpk {
+
+template <int P, int C, class R, int TX, int TY>
+struct FusedSmem {
+  static constexpr int RX2 = TX + 4, RY2 = TY + 4;  // phi region (halo 2)
+  static constexpr int RX1 = TX + 2, RY1 = TY + 2;  // mu / delta-phi region (halo 1)
+  R phi[P][RX2][RY2];
+  R mu[C][RX1][RY1];
+  R dp[P][RX1][RY1];
+};
+
+template <int P, int C, class R, int TX, int TY>
+__global__ void __launch_bounds__(256) k_fused(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                               const R* __restrict__ mu, const R* __restrict__ mulo,
+                                               R* __restrict__ phi_o, R* __restrict__ mu_o, R* __restrict__ mulo_o) {
+  using S = FusedSmem<P, C, R, TX, TY>;
+  constexpr int RX2 = S::RX2, RY2 = S::RY2, RX1 = S::RX1, RY1 = S::RY1;
+  __shared__ S sm;
+  const DevParams<R>& p = *prm;
+  const int tid = threadIdx.x;
+  const int x0 = 3 + blockIdx.y * TX;  // first interior row of the tile (padded index)
+  const int y0 = 3 + blockIdx.x * TY;
+
+  // 1. stage phi (halo 2) and mu (halo 1); indices clamped into the padded array
+  for (int i = tid; i < RX2 * RY2; i += 256) {
+    int rx = i / RY2, ry = i % RY2;
+    int gx = min(x0 - 2 + rx, p.rx - 1), gy = min(y0 - 2 + ry, p.ry - 1);
+#pragma unroll
+    for (int a = 0; a < P; a++) sm.phi[a][rx][ry] = phi[idx(p, a, gx, gy)];
+  }
+  for (int i = tid; i < RX1 * RY1; i += 256) {
+    int rx = i / RY1, ry = i % RY1;
+    int gx = min(x0 - 1 + rx, p.rx - 1), gy = min(y0 - 1 + ry, p.ry - 1);
+#pragma unroll
+    for (int k = 0; k < C; k++) sm.mu[k][rx][ry] = mu[idx(p, k, gx, gy)];
+  }
+  __syncthreads();
+
+  // 2. delta-phi on the tile plus its 1-cell ring (region cell (rx,ry) is padded
+  //    cell (x0-1+rx, y0-1+ry); its 3x3 phi block starts at phi region (rx, ry))
+  for (int i = tid; i < RX1 * RY1; i += 256) {
+    int rx = i / RY1, ry = i % RY1;
+    R v[P][3][3];
+#pragma unroll
+    for (int a = 0; a < P; a++)
+#pragma unroll
+      for (int u = 0; u < 3; u++)
+#pragma unroll
+        for (int w = 0; w < 3; w++) v[a][u][w] = sm.phi[a][rx + u][ry + w];
+    Nbhd<P, R> n;
+    make_nbhd<P>(p, v, n);
+    R m[C], dp[P];
+#pragma unroll
+    for (int k = 0; k < C; k++) m[k] = sm.mu[k][rx][ry];
+    cell_dphi<P, C>(p, n, m, dp);
+#pragma unroll
+    for (int a = 0; a < P; a++) sm.dp[a][rx][ry] = dp[a];
+  }
+  __syncthreads();
+
+  // 3. update of the tile cells
+  for (int i = tid; i < TX * TY; i += 256) {
+    int tx = i / TY, ty = i % TY;
+    int x = x0 + tx, y = y0 + ty;
+    if (x > p.mesh_x + 2 || y > p.mesh_y + 2) continue;
+    const int ax = tx + 1, ay = ty + 1;  // position in the halo-1 regions
+    R v[P][3][3];
+#pragma unroll
+    for (int a = 0; a < P; a++)
+#pragma unroll
+      for (int u = 0; u < 3; u++)
+#pragma unroll
+        for (int w = 0; w < 3; w++) v[a][u][w] = sm.phi[a][tx + 1 + u][ty + 1 + w];
+    Nbhd<P, R> n;
+    make_nbhd<P>(p, v, n);
+    R phc[P], phf[P], phb[P], phr[P], phl[P], dc[P], df[P], db[P], dr[P], dl[P];
+#pragma unroll
+    for (int a = 0; a < P; a++) {
+      phc[a] = v[a][1][1];
+      phf[a] = v[a][2][1];
+      phb[a] = v[a][0][1];
+      phr[a] = v[a][1][2];
+      phl[a] = v[a][1][0];
+      dc[a] = sm.dp[a][ax][ay];
+      df[a] = sm.dp[a][ax + 1][ay];
+      db[a] = sm.dp[a][ax - 1][ay];
+      dr[a] = sm.dp[a][ax][ay + 1];
+      dl[a] = sm.dp[a][ax][ay - 1];
+    }
+    R mc[C], mf[C], mb[C], mr[C], ml[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      mc[k] = sm.mu[k][ax][ay];
+      mf[k] = sm.mu[k][ax + 1][ay];
+      mb[k] = sm.mu[k][ax - 1][ay];
+      mr[k] = sm.mu[k][ax][ay + 1];
+      ml[k] = sm.mu[k][ax][ay - 1];
+    }
+    R jf[C], jb[C], jr[C], jl[C];
+    face_jat<P, C>(p, n.gXc, n.cYc, n.cYf, phc, phf, mc, mf, dc, df, true, jf);
+    face_jat<P, C>(p, n.gXb, n.cYb, n.cYc, phb, phc, mb, mc, db, dc, true, jb);
+    face_jat<P, C>(p, n.gYc, n.cXc, n.cXr, phc, phr, mc, mr, dc, dr, y != p.yskip, jr);
+    face_jat<P, C>(p, n.gYl, n.cXl, n.cXc, phl, phc, ml, mc, dl, dc, (y - 1) != p.yskip, jl);
+    R Dc[C][C], Df[C][C], Db[C][C], Dr[C][C], Dl[C][C];
+    cell_D<P, C>(p, phc, Dc);
+    cell_D<P, C>(p, phf, Df);
+    cell_D<P, C>(p, phb, Db);
+    cell_D<P, C>(p, phr, Dr);
+    cell_D<P, C>(p, phl, Dl);
+    R Dmf[C][C], Dmb[C][C], Dmr[C][C], Dml[C][C], gmf[C], gmb[C], gmr[C], gml[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+#pragma unroll
+      for (int l = 0; l < C; l++) {
+        Dmf[k][l] = (Df[k][l] + Dc[k][l]) * R(0.5);
+        Dmb[k][l] = (Dc[k][l] + Db[k][l]) * R(0.5);
+        Dmr[k][l] = (Dr[k][l] + Dc[k][l]) * R(0.5);
+        Dml[k][l] = (Dc[k][l] + Dl[k][l]) * R(0.5);
+      }
+      gmf[k] = div_dx(mf[k] - mc[k], p);
+      gmb[k] = div_dx(mc[k] - mb[k], p);
+      gmr[k] = div_dy(mr[k] - mc[k], p);
+      gml[k] = div_dy(mc[k] - ml[k], p);
+    }
+    R pho[P], muo[C], dmu[C];
+    cell_update<P, C>(p, phc, mc, dc, jf, jb, jr, jl, Dmf, Dmb, Dmr, Dml, gmf, gmb, gmr, gml, pho, muo, dmu);
+#pragma unroll
+    for (int a = 0; a < P; a++) write_with_ghosts(p, 0, phi_o + (size_t)a * p.plane, x, y, pho[a]);
+    if constexpr (Ops<R>::Exact) {
+#pragma unroll
+      for (int k = 0; k < C; k++) write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, muo[k]);
+    } else {
+      // compensated accumulation of the mu increment
+#pragma unroll
+      for (int k = 0; k < C; k++) {
+        R lo = mulo[idx(p, k, x, y)];
+        R yk = dmu[k] - lo;
+        R t = mc[k] + yk;
+        R lo_n = (t - mc[k]) - yk;
+        write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, t);
+        mulo_o[idx(p, k, x, y)] = lo_n;
+      }
+    }
+  }
+}
+
+}  // namespace gpk

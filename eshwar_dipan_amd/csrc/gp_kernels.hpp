@@ -138,8 +138,8 @@ __global__ void __launch_bounds__(BX * BY) k_conc(const DevParams<R>* __restrict
     gmr[k] = div_dy(mr[k] - mc[k], p);
     gml[k] = div_dy(mc[k] - ml[k], p);
   }
-  R pho[P], muo[C];
-  cell_update<P, C>(p, phc, mc, dc, jf, jb, jr, jl, Dmf, Dmb, Dmr, Dml, gmf, gmb, gmr, gml, pho, muo);
+  R pho[P], muo[C], dmu[C];
+  cell_update<P, C>(p, phc, mc, dc, jf, jb, jr, jl, Dmf, Dmb, Dmr, Dml, gmf, gmb, gmr, gml, pho, muo, dmu);
 #pragma unroll
   for (int a = 0; a < P; a++) write_with_ghosts(p, 0, phi_o + (size_t)a * p.plane, x, y, pho[a]);
 #pragma unroll

@@ -5,13 +5,35 @@
 #include <vector>
 
 #include "gp_ctx.hpp"
-#include "gp_kernels.hpp"
+#include "gp_fused.hpp"
 
 namespace gpk {
+
+// Tile of the fused kernel: the largest of (8,64), (4,64), (8,32), (4,32),
+// (2,32) interior cells whose LDS staging fits in 60 KiB.
+template <int P, int C, class R>
+struct FusedTile {
+  template <int TX, int TY>
+  static constexpr bool fits = sizeof(FusedSmem<P, C, R, TX, TY>) <= 60 * 1024;
+  static constexpr int TX = fits<8, 64> ? 8 : fits<4, 64> ? 4 : fits<8, 32> ? 8 : fits<4, 32> ? 4 : 2;
+  static constexpr int TY = fits<8, 64> || fits<4, 64> ? 64 : 32;
+};
 
 template <int P, int C, class R>
 int run_steps(gp_ctx *c, long nsteps) {
   const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
+  if (c->path == 0) {
+    using T = FusedTile<P, C, R>;
+    dim3 g((c->hp.mesh_y + T::TY - 1) / T::TY, (c->hp.mesh_x + T::TX - 1) / T::TX);
+    for (long t = 0; t < nsteps; t++) {
+      int i = c->cur, o = c->cur ^ 1;
+      hipLaunchKernelGGL((k_fused<P, C, R, T::TX, T::TY>), g, dim3(256), 0, c->stream, prm, (const R *)c->phi[i],
+                         (const R *)c->mu[i], (const R *)c->mulo[i], (R *)c->phi[o], (R *)c->mu[o], (R *)c->mulo[o]);
+      c->cur = o;
+    }
+    GP_HIP_CHECK(hipGetLastError());
+    return GP_OK;
+  }
   dim3 blk(BX, BY);
   dim3 g_phase((c->ry - 4 + BX - 1) / BX, (c->rx - 4 + BY - 1) / BY);
   dim3 g_conc((c->hp.mesh_y + BX - 1) / BX, (c->hp.mesh_x + BY - 1) / BY);

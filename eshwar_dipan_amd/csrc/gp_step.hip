@@ -209,7 +209,7 @@ int gp_create(const gp_params *p, int dtype, gp_ctx **out) {
   }
   c->stream = nullptr;
   c->cur = 0;
-  c->path = 1;
+  c->path = 0;
   *out = c;
   return GP_OK;
 }

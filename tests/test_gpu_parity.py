@@ -115,4 +115,24 @@ def test_gpu_stats_and_simplex():
     st = s.stats()
     np.testing.assert_allclose(st["phi_sum"], phi.sum(axis=(1, 2)), rtol=1e-12)
     np.testing.assert_allclose(st["mu_min"], mu.min(axis=(1, 2)), rtol=0)
-    assert np.abs(phi.sum(0) - 1.0).max() < 1e-12
+    # solute = sum_cells w * sum_a h_a(phi) c_a(mu), half weights on the first and
+    # last node of each axis, recomputed here with numpy
+    p = inp.p
+    P, C = p.nphases, p.ncomp
+    h = []
+    for a in range(P):
+        s1 = sum(phi[b] * phi[c] for b in range(P) for c in range(P) if b != a and c != a and b < c)
+        h.append(3 * phi[a] ** 2 - 2 * phi[a] ** 3 + 2 * phi[a] * s1)
+    w = np.ones(phi.shape[1:])
+    w[0, :] *= 0.5
+    w[-1, :] *= 0.5
+    w[:, 0] *= 0.5
+    w[:, -1] *= 0.5
+    for k in range(C):
+        tot = 0.0
+        for a in range(P):
+            ca = sum(p.cmu[a][k][l] * (mu[l] - (p.Beq[a][l] + p.dBbdT[a][l] * (p.T - p.Teq))) for l in range(C))
+            tot = tot + h[a] * ca
+        np.testing.assert_allclose(st["solute"][k], float((w * tot).sum()), rtol=1e-11)
+    ophi, omu = gputil.oracle_run(inp.p, phi0, mu0, 50, nthreads=0)
+    assert np.abs(phi - ophi).max() <= TOL64 and np.abs(mu - omu).max() <= TOL64
