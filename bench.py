@@ -43,26 +43,43 @@ def cpu_baseline(inp, phi0, mu0, steps, threads):
     mup[:, 3:-3, 3:-3] = mu0
     vp = lambda a: a.ctypes.data_as(ct.c_void_p)  # noqa: E731
     lib.gpo_apply_bc(ct.byref(p), vp(php), vp(mup))
-    lib.gpo_step(ct.byref(p), vp(php), vp(mup), 1, threads)  # warm caches / page in
+    t0 = time.perf_counter()
+    lib.gpo_step(ct.byref(p), vp(php), vp(mup), 1, threads)  # warm caches / page in, size the sample
+    t1 = time.perf_counter() - t0
+    if steps is None:
+        steps = max(2, min(5000, int(10.0 / max(t1, 1e-6))))
     t0 = time.perf_counter()
     rc = lib.gpo_step(ct.byref(p), vp(php), vp(mup), steps, threads)
     dt = time.perf_counter() - t0
     if rc != 0:
         raise RuntimeError(f"oracle failed: {rc}")
-    return MX * MY * steps / dt, dt
+    return MX * MY * steps / dt, dt, steps
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the workload's steps to solution, capped at 2e10 cell updates)")
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--path", type=int, default=0, help="0 fused kernel, 1 two-pass kernels")
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a plain 1 GiB device copy five times (calibration of the HBM byte counters)")
     a = ap.parse_args()
+    if a.copy_probe:
+        import torch
+        n = 1 << 28
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n, "launches": 5}))
+        return
     if a.gpus != 1:
         sys.exit(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not run "
                  "(the step shards by x-strips with 3-row halos; see DESIGN.md)")
@@ -73,9 +90,12 @@ def main():
     inp, phi0, mu0, nsteps_full, desc = workloads.WORKLOADS[a.workload]()
     p = inp.p
     P, C, MX, MY = p.nphases, p.ncomp, p.mesh_x, p.mesh_y
+    if a.steps is None:
+        a.steps = int(min(nsteps_full, max(1, 2e10 // (MX * MY))))
     s = gpstep.Solver(inp, a.dtype, path=a.path)
     s.set_state(phi0, mu0)
     s.step(a.warmup)
+    s.set_state(phi0, mu0)  # the timed steps start from the initial state
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     s.step(a.steps)
@@ -118,9 +138,7 @@ def main():
     }
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        csteps = a.cpu_steps or max(2, int(2.5e7 * (4 if P * (P - 1) > 4 else 1) // (cells * max(1, P * P // 4))))
-        csteps = min(csteps, 200)
-        v, dt = cpu_baseline(inp, phi0, mu0, csteps, threads)
+        v, dt, csteps = cpu_baseline(inp, phi0, mu0, a.cpu_steps, threads)
         out["cpu_baseline"] = {"value": v, "unit": "cell-updates/s", "cores": threads, "kind": "port",
                                "sample": f"{csteps} full steps of the same {MX}x{MY} grid "
                                          f"({dt:.1f} s, binary64 oracle, OpenMP {threads} threads)"}

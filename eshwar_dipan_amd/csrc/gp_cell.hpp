@@ -28,6 +28,7 @@ struct DevParams {
   int mesh_x, mesh_y;
   int obstacle, anisotropy, mu_solver;
   int yskip;              // padded y index MESH_Y-2 whose y-face antitrapping is skipped
+  int pow2_dx, pow2_dy;   // spacing is a power of two (division = exact multiplication)
   int gsrc[2][2][6];      // [field][axis][ghost slot] -> padded source index (slots: 0,1,2,n-3,n-2,n-1)
   R dx, dy, dt, two_dy;
   R inv_dx, inv_dy, inv_two_dy;
@@ -61,17 +62,26 @@ struct Ops<float> {
   static constexpr bool Exact = false;
 };
 
+// Division by a grid spacing.  When the spacing is a power of two, v/d and
+// v*(1/d) round the same real number, so the multiplication is exact; the
+// binary64 path divides otherwise.
 template <class R>
 __device__ __forceinline__ R div_dx(R v, const DevParams<R>& p) {
-  if constexpr (Ops<R>::Exact) return v / p.dx; else return v * p.inv_dx;
+  if constexpr (Ops<R>::Exact) return p.pow2_dx ? v * p.inv_dx : v / p.dx; else return v * p.inv_dx;
 }
 template <class R>
 __device__ __forceinline__ R div_dy(R v, const DevParams<R>& p) {
-  if constexpr (Ops<R>::Exact) return v / p.dy; else return v * p.inv_dy;
+  if constexpr (Ops<R>::Exact) return p.pow2_dy ? v * p.inv_dy : v / p.dy; else return v * p.inv_dy;
 }
 template <class R>
 __device__ __forceinline__ R div_2dy(R v, const DevParams<R>& p) {
-  if constexpr (Ops<R>::Exact) return v / p.two_dy; else return v * p.inv_two_dy;
+  if constexpr (Ops<R>::Exact) return p.pow2_dy ? v * p.inv_two_dy : v / p.two_dy; else return v * p.inv_two_dy;
+}
+
+// sqrt that skips the evaluation for an exact zero argument (sqrt(+0) = +0)
+template <class R>
+__device__ __forceinline__ R sqrt_nz(R v) {
+  return v == R(0) ? R(0) : sqrt(v);
 }
 
 // ---------------------------------------------------------------- thermodynamics
@@ -515,11 +525,11 @@ __device__ __forceinline__ void face_jat(const DevParams<R>& p, const R* g_p, co
   for (int k = 0; k < C; k++) jat[k] = R(0);
   if (!enabled) return;
   R tl = R(0.5) * (ct_p[L] + ct_q[L]);
-  R ngl = sqrt(tl * tl + g_p[L] * g_p[L]);
+  R ngl = sqrt_nz(tl * tl + g_p[L] * g_p[L]);
 #pragma unroll
   for (int a = 0; a < L; a++) {
     R t = R(0.5) * (ct_p[a] + ct_q[a]);
-    R ngp = sqrt(t * t + g_p[a] * g_p[a]);
+    R ngp = sqrt_nz(t * t + g_p[a] * g_p[a]);
     if (ngp != R(0)) {
       R s_q = R(0), s_p = R(0);
       if (p.obstacle) {

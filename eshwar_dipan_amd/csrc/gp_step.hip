@@ -41,6 +41,11 @@ static void fill_devparams(const gp_params &h, int rx, int ry, int pitch, long p
   d.anisotropy = h.anisotropy;
   d.mu_solver = h.mu_solver;
   d.yskip = (int)h.mesh_y - 2;
+  {
+    int e;
+    d.pow2_dx = frexp(h.dx, &e) == 0.5;
+    d.pow2_dy = frexp(h.dy, &e) == 0.5;
+  }
   for (int f = 0; f < 2; f++) {
     // slots 0,1,2 = ghosts 0,1,2 (faces X- / Y-), slots 3,4,5 = ghosts n-3, n-2, n-1 (X+ / Y+)
     d.gsrc[f][0][0] = (int)h.bc_proxy[f][GP_FACE_XMIN][2];
