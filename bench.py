@@ -64,7 +64,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
-    ap.add_argument("--path", type=int, default=0, help="0 fused kernel, 1 two-pass kernels")
+    ap.add_argument("--path", type=int, default=0, help="0 sparse interface scheduling, 1 two-pass, 2 dense fused")
+    ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -92,7 +93,7 @@ def main():
     P, C, MX, MY = p.nphases, p.ncomp, p.mesh_x, p.mesh_y
     if a.steps is None:
         a.steps = int(min(nsteps_full, max(1, 2e10 // (MX * MY))))
-    s = gpstep.Solver(inp, a.dtype, path=a.path)
+    s = gpstep.Solver(inp, a.dtype, path=a.path, graphs=not a.no_graphs)
     s.set_state(phi0, mu0)
     s.step(a.warmup)
     s.set_state(phi0, mu0)  # the timed steps start from the initial state
@@ -130,7 +131,7 @@ def main():
         "dtype": a.dtype,
         "data": "synthetic (seeded microstructure)",
         "config": {"workload": desc, "mesh_x": MX, "mesh_y": MY, "phases": P, "components": C + 1,
-                   "path": "fused" if a.path == 0 else "two-pass", "steps_to_solution": nsteps_full},
+                   "path": ["sparse interface tiles (fused LDS kernel) + bulk tiles", "two-pass", "dense fused"][a.path], "steps_to_solution": nsteps_full},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_cell_update": B, "kernel_ms_per_step": kern_s * 1e3},

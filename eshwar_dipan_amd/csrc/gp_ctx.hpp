@@ -25,7 +25,25 @@ struct gp_ctx {
   hipStream_t stream;
   hipEvent_t ev0, ev1;
   double last_ms;
+  // sparse interface scheduling (fused path)
+  int tile_tx, tile_ty;  // tile shape of the instantiated fused kernels
+  int ntx, nty, ntiles, maxdep;
+  int *dflag[2];         // per-tile one-hot flag of state buffers 0/1 (-1: not bulk)
+  int *ddeps;            // [ntiles][maxdep+1]: count, tiles the 2-cell halo reads from
+  int *dlist;            // [2][ntiles]: interface list, bulk list
+  int *dlist_all;        // all tiles (dense fused path)
+  int *dcnt;             // [2 step parities][interface, bulk] counters; [4],[5] = ntiles
+  int cnt_par;
+  int flags_valid;
+  long n_if_last, n_bulk_last;
+  // step graphs: GRAPH_STEPS steps captured once per starting parity
+  hipStream_t own_stream;   // created with default priority when the caller sets none
+  hipGraphExec_t graph[2];
+  int graph_path;
+  int use_graphs;
 };
+
+#define GP_GRAPH_STEPS 32
 
 namespace gpk {
 

@@ -29,17 +29,52 @@ struct FusedSmem {
   R dp[P][RX1][RY1];
 };
 
+// Bulk-tile flag of a cell: the phase a if phi is one-hot in a (phi_a == 1, all
+// others == 0), -1 otherwise; -2 marks "no cell" in reductions.
+template <int P, class R>
+__device__ __forceinline__ int onehot_phase(const R* ph) {
+  int a1 = -1, nz = 0;
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    if (ph[a] == R(1.0)) a1 = a;
+    else if (ph[a] != R(0)) nz++;
+  }
+  return nz == 0 ? a1 : -1;
+}
+
+__device__ __forceinline__ int flag_combine(int x, int y) { return x == -2 ? y : y == -2 ? x : (x == y ? x : -1); }
+
+// Block-wide combination of per-thread flags; result valid in thread 0.
+__device__ __forceinline__ int block_flag(int f, int* red) {
+  red[threadIdx.x] = f;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = flag_combine(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  int r = red[0];
+  __syncthreads();
+  return r;
+}
+
 template <int P, int C, class R, int TX, int TY>
 __global__ void __launch_bounds__(256) k_fused(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                const R* __restrict__ mu, const R* __restrict__ mulo,
-                                               R* __restrict__ phi_o, R* __restrict__ mu_o, R* __restrict__ mulo_o) {
+                                               R* __restrict__ phi_o, R* __restrict__ mu_o, R* __restrict__ mulo_o,
+                                               const int* __restrict__ list, const int* __restrict__ count,
+                                               int* __restrict__ flag_o, int nty) {
   using S = FusedSmem<P, C, R, TX, TY>;
   constexpr int RX2 = S::RX2, RY2 = S::RY2, RX1 = S::RX1, RY1 = S::RY1;
   __shared__ S sm;
+  __shared__ int red[256];
   const DevParams<R>& p = *prm;
   const int tid = threadIdx.x;
-  const int x0 = 3 + blockIdx.y * TX;  // first interior row of the tile (padded index)
-  const int y0 = 3 + blockIdx.x * TY;
+  const int ntile = *count;
+  for (int it = blockIdx.x; it < ntile; it += gridDim.x) {
+  const int tile = list[it];
+  const int x0 = 3 + (tile / nty) * TX;  // first interior row of the tile (padded index)
+  const int y0 = 3 + (tile % nty) * TY;
+  int myflag = -2;
 
   // 1. stage phi (halo 2) and mu (halo 1); indices clamped into the padded array
   for (int i = tid; i < RX2 * RY2; i += 256) {
@@ -144,6 +179,7 @@ __global__ void __launch_bounds__(256) k_fused(const DevParams<R>* __restrict__ 
     }
     R pho[P], muo[C], dmu[C];
     cell_update<P, C>(p, phc, mc, dc, jf, jb, jr, jl, Dmf, Dmb, Dmr, Dml, gmf, gmb, gmr, gml, pho, muo, dmu);
+    myflag = flag_combine(myflag, onehot_phase<P>(pho));
 #pragma unroll
     for (int a = 0; a < P; a++) write_with_ghosts(p, 0, phi_o + (size_t)a * p.plane, x, y, pho[a]);
     if constexpr (Ops<R>::Exact) {
@@ -162,6 +198,122 @@ __global__ void __launch_bounds__(256) k_fused(const DevParams<R>* __restrict__ 
       }
     }
   }
+  int f = block_flag(myflag, red);
+  if (tid == 0) flag_o[tile] = f == -2 ? -1 : f;
+  }  // tile loop
+}
+
+// Bulk tiles: the tile and its 2-cell halo are one-hot in one phase, so every
+// delta-phi and antitrapping flux the full step would compute for these cells is
+// an exact zero and every cell takes the bulk branch of the mu update with the
+// same mobility.  The update below passes those zeros through the same
+// cell_update, so the results equal the full kernel's bit for bit; phi is copied.
+template <int P, int C, class R, int TX, int TY>
+__global__ void __launch_bounds__(256) k_bulk(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                              const R* __restrict__ mu, const R* __restrict__ mulo,
+                                              R* __restrict__ phi_o, R* __restrict__ mu_o, R* __restrict__ mulo_o,
+                                              const int* __restrict__ list, const int* __restrict__ count,
+                                              const int* __restrict__ flag_i, int* __restrict__ flag_o, int nty) {
+  const DevParams<R>& p = *prm;
+  const int ntile = *count;
+  for (int it = blockIdx.x; it < ntile; it += gridDim.x) {
+    const int tile = list[it];
+    const int x0 = 3 + (tile / nty) * TX;
+    const int y0 = 3 + (tile % nty) * TY;
+    for (int i = threadIdx.x; i < TX * TY; i += 256) {
+      const int x = x0 + i / TY, y = y0 + i % TY;
+      if (x > p.mesh_x + 2 || y > p.mesh_y + 2) continue;
+      R ph[P], zp[P], mc[C], mf[C], mb[C], mr[C], ml[C], z[C];
+#pragma unroll
+      for (int a = 0; a < P; a++) {
+        ph[a] = phi[idx(p, a, x, y)];
+        zp[a] = R(0);
+      }
+#pragma unroll
+      for (int k = 0; k < C; k++) {
+        mc[k] = mu[idx(p, k, x, y)];
+        mf[k] = mu[idx(p, k, x + 1, y)];
+        mb[k] = mu[idx(p, k, x - 1, y)];
+        mr[k] = mu[idx(p, k, x, y + 1)];
+        ml[k] = mu[idx(p, k, x, y - 1)];
+        z[k] = R(0);
+      }
+      // the five cells carry the same phi, hence the same mobility
+      R D[C][C], Dm[C][C], gmf[C], gmb[C], gmr[C], gml[C];
+      cell_D<P, C>(p, ph, D);
+#pragma unroll
+      for (int k = 0; k < C; k++) {
+#pragma unroll
+        for (int l = 0; l < C; l++) Dm[k][l] = (D[k][l] + D[k][l]) * R(0.5);
+        gmf[k] = div_dx(mf[k] - mc[k], p);
+        gmb[k] = div_dx(mc[k] - mb[k], p);
+        gmr[k] = div_dy(mr[k] - mc[k], p);
+        gml[k] = div_dy(mc[k] - ml[k], p);
+      }
+      R pho[P], muo[C], dmu[C];
+      cell_update<P, C>(p, ph, mc, zp, z, z, z, z, Dm, Dm, Dm, Dm, gmf, gmb, gmr, gml, pho, muo, dmu);
+#pragma unroll
+      for (int a = 0; a < P; a++) write_with_ghosts(p, 0, phi_o + (size_t)a * p.plane, x, y, pho[a]);
+      if constexpr (Ops<R>::Exact) {
+#pragma unroll
+        for (int k = 0; k < C; k++) write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, muo[k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+          R lo = mulo[idx(p, k, x, y)];
+          R yk = dmu[k] - lo;
+          R t = mc[k] + yk;
+          R lo_n = (t - mc[k]) - yk;
+          write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, t);
+          mulo_o[idx(p, k, x, y)] = lo_n;
+        }
+      }
+    }
+    if (threadIdx.x == 0) flag_o[tile] = flag_i[tile];
+  }
+}
+
+// Tile classification for one step: a tile is bulk when it and every tile its
+// 2-cell halo reads from (deps, through the ghost copies) carry the same one-hot
+// flag.  Appends to the interface / bulk lists with atomics on cnt[0..1] and
+// clears the counters of the next step (cnt_next).
+template <int Unused = 0>
+__global__ void k_plan(const int* __restrict__ flag, const int* __restrict__ deps, int maxdep, int ntiles,
+                       int* __restrict__ list_if, int* __restrict__ list_bulk, int* __restrict__ cnt,
+                       int* __restrict__ cnt_next) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    cnt_next[0] = 0;
+    cnt_next[1] = 0;
+  }
+  if (t >= ntiles) return;
+  const int* d = deps + (size_t)t * (maxdep + 1);
+  int f = flag[t], nd = d[0];
+  bool bulk = f >= 0 && nd >= 0;
+  for (int j = 1; bulk && j <= nd; j++) bulk = flag[d[j]] == f;
+  if (bulk) list_bulk[atomicAdd(&cnt[1], 1)] = t;
+  else list_if[atomicAdd(&cnt[0], 1)] = t;
+}
+
+// Flags of a freshly loaded state.
+template <int P, class R, int TX, int TY>
+__global__ void __launch_bounds__(256) k_flags(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                               int* __restrict__ flag, int nty) {
+  __shared__ int red[256];
+  const DevParams<R>& p = *prm;
+  const int tile = blockIdx.x;
+  const int x0 = 3 + (tile / nty) * TX, y0 = 3 + (tile % nty) * TY;
+  int f = -2;
+  for (int i = threadIdx.x; i < TX * TY; i += 256) {
+    const int x = x0 + i / TY, y = y0 + i % TY;
+    if (x > p.mesh_x + 2 || y > p.mesh_y + 2) continue;
+    R ph[P];
+#pragma unroll
+    for (int a = 0; a < P; a++) ph[a] = phi[idx(p, a, x, y)];
+    f = flag_combine(f, onehot_phase<P>(ph));
+  }
+  f = block_flag(f, red);
+  if (threadIdx.x == 0) flag[tile] = f == -2 ? -1 : f;
 }
 
 }  // namespace gpk

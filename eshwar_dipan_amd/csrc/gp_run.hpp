@@ -2,6 +2,7 @@
 // the instantiation files gp_inst_p*.hip.
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include "gp_ctx.hpp"
@@ -19,17 +20,109 @@ struct FusedTile {
   static constexpr int TY = fits<8, 64> || fits<4, 64> ? 64 : 32;
 };
 
+// Host-side tile geometry and ghost-aware dependency lists: tile t depends on
+// every tile owning a source cell of its 2-cell halo, the halo's ghost cells
+// mapped to their copy sources (bc_proxy of phi).
+inline int setup_tiles(gp_ctx *c, int TX, int TY) {
+  if (c->dflag[0] && c->tile_tx == TX && c->tile_ty == TY) return GP_OK;
+  const long MX = c->hp.mesh_x, MY = c->hp.mesh_y;
+  c->tile_tx = TX;
+  c->tile_ty = TY;
+  c->ntx = (int)((MX + TX - 1) / TX);
+  c->nty = (int)((MY + TY - 1) / TY);
+  c->ntiles = c->ntx * c->nty;
+  c->maxdep = 16;
+  const int rows[2] = {c->rx, c->ry};
+  // ghost source along an axis: padded index -> interior padded index
+  auto src = [&](int axis, int v) -> int {
+    long n = axis ? MY : MX;
+    if (v >= 3 && v <= n + 2) return v;
+    const long *lo = c->hp.bc_proxy[0][axis ? GP_FACE_YMIN : GP_FACE_XMIN];
+    const long *hi = c->hp.bc_proxy[0][axis ? GP_FACE_YMAX : GP_FACE_XMAX];
+    if (v <= 2) return (int)lo[2 - v];                 // ghosts 2,1,0 <- proxy 0,1,2
+    return (int)hi[v - (rows[axis] - 3)];              // ghosts n-3.. <- proxy 0,1,2
+  };
+  std::vector<int> deps((size_t)c->ntiles * (c->maxdep + 1), 0);
+  for (int tx = 0; tx < c->ntx; tx++)
+    for (int ty = 0; ty < c->nty; ty++) {
+      int sx[64], sy[128], nsx = 0, nsy = 0;
+      int x0 = 3 + tx * TX, x1 = (int)std::min<long>(x0 + TX - 1, MX + 2);
+      int y0 = 3 + ty * TY, y1 = (int)std::min<long>(y0 + TY - 1, MY + 2);
+      for (int x = x0 - 2; x <= x1 + 2; x++) {
+        int o = (src(0, x) - 3) / TX;
+        bool seen = false;
+        for (int i = 0; i < nsx; i++) seen |= sx[i] == o;
+        if (!seen) sx[nsx++] = o;
+      }
+      for (int y = y0 - 2; y <= y1 + 2; y++) {
+        int o = (src(1, y) - 3) / TY;
+        bool seen = false;
+        for (int i = 0; i < nsy; i++) seen |= sy[i] == o;
+        if (!seen && nsy < 128) sy[nsy++] = o;
+      }
+      int *d = &deps[(size_t)(tx * c->nty + ty) * (c->maxdep + 1)];
+      if (nsx * nsy > c->maxdep) {
+        d[0] = -1;  // always an interface tile
+        continue;
+      }
+      int n = 0;
+      for (int i = 0; i < nsx; i++)
+        for (int j = 0; j < nsy; j++) d[1 + n++] = sx[i] * c->nty + sy[j];
+      d[0] = n;
+    }
+  std::vector<int> all(c->ntiles);
+  for (int i = 0; i < c->ntiles; i++) all[i] = i;
+  size_t nt = (size_t)c->ntiles;
+  GP_HIP_CHECK(hipMalloc((void **)&c->dflag[0], nt * sizeof(int)));
+  GP_HIP_CHECK(hipMalloc((void **)&c->dflag[1], nt * sizeof(int)));
+  GP_HIP_CHECK(hipMalloc((void **)&c->ddeps, deps.size() * sizeof(int)));
+  GP_HIP_CHECK(hipMalloc((void **)&c->dlist, 2 * nt * sizeof(int)));
+  GP_HIP_CHECK(hipMalloc((void **)&c->dlist_all, nt * sizeof(int)));
+  GP_HIP_CHECK(hipMalloc((void **)&c->dcnt, 8 * sizeof(int)));
+  int cnt0[8] = {0, 0, 0, 0, c->ntiles, c->ntiles, 0, 0};
+  GP_HIP_CHECK(hipMemcpy(c->ddeps, deps.data(), deps.size() * sizeof(int), hipMemcpyHostToDevice));
+  GP_HIP_CHECK(hipMemcpy(c->dlist_all, all.data(), nt * sizeof(int), hipMemcpyHostToDevice));
+  GP_HIP_CHECK(hipMemcpy(c->dcnt, cnt0, sizeof(cnt0), hipMemcpyHostToDevice));
+  c->cnt_par = 0;
+  c->flags_valid = 0;
+  return GP_OK;
+}
+
 template <int P, int C, class R>
 int run_steps(gp_ctx *c, long nsteps) {
   const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
-  if (c->path == 0) {
+  if (c->path == 0 || c->path == 2) {
     using T = FusedTile<P, C, R>;
-    dim3 g((c->hp.mesh_y + T::TY - 1) / T::TY, (c->hp.mesh_x + T::TX - 1) / T::TX);
+    int rc = setup_tiles(c, T::TX, T::TY);
+    if (rc) return rc;
+    const int nt = c->ntiles;
+    if (!c->flags_valid) {
+      hipLaunchKernelGGL((k_flags<P, R, T::TX, T::TY>), dim3(nt), dim3(256), 0, c->stream, prm,
+                         (const R *)c->phi[c->cur], c->dflag[c->cur], c->nty);
+      c->flags_valid = 1;
+    }
+    const int g_if = std::min(nt, 2048), g_bulk = std::min(nt, 4096);
     for (long t = 0; t < nsteps; t++) {
       int i = c->cur, o = c->cur ^ 1;
-      hipLaunchKernelGGL((k_fused<P, C, R, T::TX, T::TY>), g, dim3(256), 0, c->stream, prm, (const R *)c->phi[i],
-                         (const R *)c->mu[i], (const R *)c->mulo[i], (R *)c->phi[o], (R *)c->mu[o], (R *)c->mulo[o]);
+      int *cnt = c->dcnt + 2 * c->cnt_par, *cnt_next = c->dcnt + 2 * (c->cnt_par ^ 1);
+      const int *list_if = c->dlist, *cnt_if = cnt;
+      if (c->path == 0) {
+        hipLaunchKernelGGL(k_plan<0>, dim3((nt + 255) / 256), dim3(256), 0, c->stream, (const int *)c->dflag[i],
+                           (const int *)c->ddeps, c->maxdep, nt, c->dlist, c->dlist + nt, cnt, cnt_next);
+      } else {
+        list_if = c->dlist_all;
+        cnt_if = c->dcnt + 4;
+      }
+      hipLaunchKernelGGL((k_fused<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, prm,
+                         (const R *)c->phi[i], (const R *)c->mu[i], (const R *)c->mulo[i], (R *)c->phi[o],
+                         (R *)c->mu[o], (R *)c->mulo[o], list_if, cnt_if, c->dflag[o], c->nty);
+      if (c->path == 0)
+        hipLaunchKernelGGL((k_bulk<P, C, R, T::TX, T::TY>), dim3(g_bulk), dim3(256), 0, c->stream, prm,
+                           (const R *)c->phi[i], (const R *)c->mu[i], (const R *)c->mulo[i], (R *)c->phi[o],
+                           (R *)c->mu[o], (R *)c->mulo[o], (const int *)(c->dlist + nt), (const int *)(cnt + 1),
+                           (const int *)c->dflag[i], c->dflag[o], c->nty);
       c->cur = o;
+      c->cnt_par ^= 1;
     }
     GP_HIP_CHECK(hipGetLastError());
     return GP_OK;
@@ -44,6 +137,7 @@ int run_steps(gp_ctx *c, long nsteps) {
     hipLaunchKernelGGL((k_conc<P, C, R>), g_conc, blk, 0, c->stream, prm, phi, mu, (const R *)c->dphi, phi_o, mu_o);
     c->cur ^= 1;
   }
+  c->flags_valid = 0;
   GP_HIP_CHECK(hipGetLastError());
   return GP_OK;
 }

@@ -111,6 +111,7 @@ static int load_state(gp_ctx *c, const S *dphi_src, const S *dmu_src) {
   hipLaunchKernelGGL((gpk::k_ghosts<R>), gi, blk, 0, c->stream, prm, (R *)c->phi[c->cur], (R *)c->mu[c->cur], c->P, c->C);
   if (c->mulo[c->cur]) GP_HIP_CHECK(hipMemsetAsync(c->mulo[c->cur], 0, (size_t)c->plane * c->esize * c->C, c->stream));
   GP_HIP_CHECK(hipGetLastError());
+  c->flags_valid = 0;
   return GP_OK;
 }
 
@@ -150,6 +151,15 @@ static void free_ctx(gp_ctx *c) {
   if (c->dphi) (void)hipFree(c->dphi);
   if (c->dprm) (void)hipFree(c->dprm);
   if (c->dpart) (void)hipFree(c->dpart);
+  for (int i = 0; i < 2; i++)
+    if (c->dflag[i]) (void)hipFree(c->dflag[i]);
+  if (c->ddeps) (void)hipFree(c->ddeps);
+  if (c->dlist) (void)hipFree(c->dlist);
+  if (c->dlist_all) (void)hipFree(c->dlist_all);
+  if (c->dcnt) (void)hipFree(c->dcnt);
+  for (int i = 0; i < 2; i++)
+    if (c->graph[i]) (void)hipGraphExecDestroy(c->graph[i]);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   delete c;
@@ -212,9 +222,16 @@ int gp_create(const gp_params *p, int dtype, gp_ctx **out) {
     free_ctx(c);
     return GP_E_NOMEM;
   }
-  c->stream = nullptr;
+  // a stream of our own (default priority) so that step blocks can be captured into graphs
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    free_ctx(c);
+    return GP_E_HIP;
+  }
+  c->stream = c->own_stream;
   c->cur = 0;
   c->path = 0;
+  c->use_graphs = 1;
+  c->graph_path = -1;
   *out = c;
   return GP_OK;
 }
@@ -227,12 +244,18 @@ void gp_destroy(gp_ctx *c) {
 
 int gp_set_stream(gp_ctx *c, void *s) {
   if (!c) return GP_E_ARG;
-  c->stream = (hipStream_t)s;
+  c->stream = s ? (hipStream_t)s : c->own_stream;
+  return GP_OK;
+}
+
+int gp_set_graphs(gp_ctx *c, int on) {
+  if (!c) return GP_E_ARG;
+  c->use_graphs = on != 0;
   return GP_OK;
 }
 
 int gp_set_path(gp_ctx *c, int path) {
-  if (!c || path < 0 || path > 1) return GP_E_ARG;
+  if (!c || path < 0 || path > 2) return GP_E_ARG;
   c->path = path;
   return GP_OK;
 }
@@ -278,15 +301,60 @@ int gp_get_state_device(gp_ctx *c, void *phi, void *mu) {
                             : store_state<float, float>(c, (float *)phi, (float *)mu);
 }
 
-int gp_step(gp_ctx *c, long nsteps) {
-  if (!c || nsteps < 0) return GP_E_ARG;
-  GP_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+static int dispatch_steps(gp_ctx *c, long nsteps) {
   int rc = GP_E_UNSUPPORTED;
 #define GP_DISPATCH_STEP(P_, C_)                                                                   \
   if (c->P == P_ && c->C == C_)                                                                    \
     rc = c->dtype == GP_F64 ? gpk::run_steps<P_, C_, double>(c, nsteps) : gpk::run_steps<P_, C_, float>(c, nsteps);
   GP_PC_LIST(GP_DISPATCH_STEP)
 #undef GP_DISPATCH_STEP
+  return rc;
+}
+
+// Capture GP_GRAPH_STEPS steps starting from the current buffer parity.  The
+// block length is even, so after it the parities (state buffers, tile flags,
+// counters) are back where they started and the graph can be replayed.
+static int capture_graph(gp_ctx *c) {
+  int slot = c->cur;
+  if (c->graph[slot] && c->graph_path == c->path) return GP_OK;
+  if (c->graph_path != c->path) {
+    for (int i = 0; i < 2; i++)
+      if (c->graph[i]) {
+        (void)hipGraphExecDestroy(c->graph[i]);
+        c->graph[i] = nullptr;
+      }
+    c->graph_path = c->path;
+  }
+  int cur0 = c->cur, par0 = c->cnt_par;
+  hipGraph_t g;
+  GP_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = dispatch_steps(c, GP_GRAPH_STEPS);
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->cur = cur0;
+  c->cnt_par = par0;
+  if (rc) return rc;
+  if (e != hipSuccess) return GP_E_HIP;
+  e = hipGraphInstantiate(&c->graph[slot], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  return e == hipSuccess ? GP_OK : GP_E_HIP;
+}
+
+int gp_step(gp_ctx *c, long nsteps) {
+  if (!c || nsteps < 0) return GP_E_ARG;
+  GP_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+  int rc = GP_OK;
+  long left = nsteps;
+  // graphs: fused paths, on a capturable stream, for runs of at least one block
+  if (c->use_graphs && c->path != 1 && c->stream && left >= GP_GRAPH_STEPS) {
+    rc = dispatch_steps(c, 0);  // tile set-up and flags of a fresh state, outside the capture
+    while (!rc && left >= GP_GRAPH_STEPS) {
+      rc = capture_graph(c);
+      if (rc) break;
+      if (hipGraphLaunch(c->graph[c->cur], c->stream) != hipSuccess) return GP_E_HIP;
+      left -= GP_GRAPH_STEPS;  // parities unchanged after an even block
+    }
+  }
+  if (!rc && left > 0) rc = dispatch_steps(c, left);
   if (rc) return rc;
   GP_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
   GP_HIP_CHECK(hipEventSynchronize(c->ev1));

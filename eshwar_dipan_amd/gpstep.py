@@ -41,7 +41,7 @@ def lib():
             ("gp_set_state_device", [vp, vp, vp]), ("gp_get_state_device", [vp, vp, vp]),
             ("gp_step", [vp, l]), ("gp_stats_get", [vp, ct.POINTER(GpStats)]),
             ("gp_set_stream", [vp, vp]), ("gp_last_step_ms", [vp, ct.POINTER(d)]),
-            ("gp_set_path", [vp, i]),
+            ("gp_set_path", [vp, i]), ("gp_set_graphs", [vp, i]),
             ("gp_read_input", [ct.c_char_p, ct.POINTER(GpInput), i, ct.c_char_p, ct.c_size_t]),
             ("gp_derive", [ct.POINTER(GpParams)]),
             ("gp_check_supported", [ct.POINTER(GpParams), ct.c_char_p, ct.c_size_t]),
@@ -121,7 +121,7 @@ class Solver:
     (binary32 bulk stencil, compensated mu, binary64 reductions).
     """
 
-    def __init__(self, params, dtype="f64", path=None):
+    def __init__(self, params, dtype="f64", path=None, graphs=True):
         if isinstance(params, GpInput):
             params = params.p
         self.params = params
@@ -131,6 +131,7 @@ class Solver:
         self._h = h
         if path is not None:
             _check(lib().gp_set_path(h, int(path)), "gp_set_path")
+        _check(lib().gp_set_graphs(h, int(bool(graphs))), "gp_set_graphs")
 
     def close(self):
         if self._h:

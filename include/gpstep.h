@@ -153,9 +153,16 @@ int  gp_set_stream(gp_ctx *ctx, void *hip_stream);
    context's stream, bracketing every kernel of that call). */
 int  gp_last_step_ms(gp_ctx *ctx, double *ms);
 
-/* Kernel path of a context: 0 = auto (fused LDS kernel), 1 = two-pass reference
-   kernels (phase kernel + concentration kernel). */
+/* Kernel path of a context: 0 = fused LDS kernel on interface tiles plus a
+   diffusion-only kernel on bulk tiles (default), 1 = two-pass kernels (phase
+   kernel + concentration kernel over the whole grid), 2 = fused LDS kernel on
+   every tile.  All paths give bit-identical binary64 results. */
 int  gp_set_path(gp_ctx *ctx, int path);
+
+/* Replay gp_step calls of 32 steps or more as captured HIP graphs of 32-step
+   blocks (default on; paths 0 and 2).  Capture needs a non-default stream: the
+   context owns one unless gp_set_stream gave another. */
+int  gp_set_graphs(gp_ctx *ctx, int on);
 
 const char *gp_strerror(int code);
 
