@@ -103,6 +103,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     dev_ms = s.last_step_ms()
+    sched = s.sched_info() if a.path != 1 else None
     st = s.stats(allow_nonfinite=True)
     cells = MX * MY
     value = cells * a.steps / wall
@@ -137,6 +138,8 @@ def main():
                      "bytes_per_cell_update": B, "kernel_ms_per_step": kern_s * 1e3},
         "nonfinite": st["nonfinite"],
     }
+    if sched:
+        out["tiles_last_step"] = {"total": sched[0], "interface": sched[1], "bulk": sched[2]}
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         v, dt, csteps = cpu_baseline(inp, phi0, mu0, a.cpu_steps, threads)

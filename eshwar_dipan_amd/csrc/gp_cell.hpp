@@ -237,12 +237,19 @@ struct Nbhd {
   R cXc[P], cYc[P];        // centred gradients at the cell
   R cYf[P], cYb[P];        // centred y-gradients at (x+1,y), (x-1,y)
   R cXr[P], cXl[P];        // centred x-gradients at (x,y+1), (x,y-1)
+  bool present[P];         // phase b is nonzero somewhere in the 3x3 block
 };
 
 template <int P, class R>
 __device__ __forceinline__ void make_nbhd(const DevParams<R>& p, const R (&v)[P][3][3], Nbhd<P, R>& n) {
 #pragma unroll
   for (int a = 0; a < P; a++) {
+    bool nz = false;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) nz = nz || (v[a][i][j] != R(0));
+    n.present[a] = nz;
     n.ph[a] = v[a][1][1];
     n.gXc[a] = div_dx(v[a][2][1] - v[a][1][1], p);
     n.gXb[a] = div_dx(v[a][1][1] - v[a][0][1], p);
@@ -272,7 +279,7 @@ __device__ __forceinline__ R dAdphi(const DevParams<R>& p, const Nbhd<P, R>& n, 
     R sum = R(0);
 #pragma unroll
     for (int b = 0; b < P; b++) {
-      if (b == a) continue;
+      if (b == a || !n.present[b]) continue;  // an absent phase contributes exact zeros
       R sp = R(0);
       sp += (n.ph[a] * n.cXc[b] - n.ph[b] * n.cXc[a]) * n.cXc[b];
       sp += (n.ph[a] * n.cYc[b] - n.ph[b] * n.cYc[a]) * n.cYc[b];
@@ -284,7 +291,7 @@ __device__ __forceinline__ R dAdphi(const DevParams<R>& p, const Nbhd<P, R>& n, 
   R sum1 = R(0), sum2 = R(0);
 #pragma unroll
   for (int b = 0; b < P; b++) {
-    if (b == a) continue;
+    if (b == a || !n.present[b]) continue;  // an absent phase contributes exact zeros
     R u[2], q[2], d[2], dq[2], rdq[2];
     u[0] = (n.ph[a] * n.cXc[b] - n.ph[b] * n.cXc[a]);
     u[1] = (n.ph[a] * n.cYc[b] - n.ph[b] * n.cYc[a]);
@@ -336,7 +343,7 @@ __device__ __forceinline__ R divdAdgradphi(const DevParams<R>& p, const Nbhd<P, 
     R sum = R(0);
 #pragma unroll
     for (int b = 0; b < P; b++) {
-      if (b == a) continue;
+      if (b == a || !n.present[b]) continue;  // an absent phase contributes exact zeros
       R fc = n.sXc[a] * n.gXc[b] - n.sXc[b] * n.gXc[a];
       R fb = n.sXb[a] * n.gXb[b] - n.sXb[b] * n.gXb[a];
       R dv = div_dx(fc * (-n.sXc[b]) - (fb * (-n.sXb[b])), p);
@@ -351,7 +358,7 @@ __device__ __forceinline__ R divdAdgradphi(const DevParams<R>& p, const Nbhd<P, 
   R sum1 = R(0), sum2 = R(0);
 #pragma unroll
   for (int b = 0; b < P; b++) {
-    if (b == a) continue;
+    if (b == a || !n.present[b]) continue;  // an absent phase contributes exact zeros
     FaceQ<R> F, Bk, Rt, L;
     face_q(p, a, b, n.sXc[a] * n.gXc[b] - n.sXc[b] * n.gXc[a],
            R(0.5) * (n.sXc[a] * (n.cYc[b] + n.cYf[b]) - n.sXc[b] * (n.cYc[a] + n.cYf[a])),

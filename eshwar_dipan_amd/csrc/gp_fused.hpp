@@ -286,13 +286,29 @@ __global__ void k_plan(const int* __restrict__ flag, const int* __restrict__ dep
     cnt_next[0] = 0;
     cnt_next[1] = 0;
   }
-  if (t >= ntiles) return;
-  const int* d = deps + (size_t)t * (maxdep + 1);
-  int f = flag[t], nd = d[0];
-  bool bulk = f >= 0 && nd >= 0;
-  for (int j = 1; bulk && j <= nd; j++) bulk = flag[d[j]] == f;
-  if (bulk) list_bulk[atomicAdd(&cnt[1], 1)] = t;
-  else list_if[atomicAdd(&cnt[0], 1)] = t;
+  const bool valid = t < ntiles;
+  bool bulk = false;
+  if (valid) {
+    const int* d = deps + (size_t)t * (maxdep + 1);
+    int f = flag[t], nd = d[0];
+    bulk = f >= 0 && nd >= 0;
+    for (int j = 1; bulk && j <= nd; j++) bulk = flag[d[j]] == f;
+  }
+  // one atomic per wave and list: lanes take consecutive slots
+  const unsigned long long mb = __ballot(valid && bulk), mi = __ballot(valid && !bulk);
+  const int lane = threadIdx.x & 63;
+  int bb = 0, bi = 0;
+  if (lane == 0) {
+    if (mb) bb = atomicAdd(&cnt[1], __popcll(mb));
+    if (mi) bi = atomicAdd(&cnt[0], __popcll(mi));
+  }
+  bb = __shfl(bb, 0);
+  bi = __shfl(bi, 0);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  if (valid) {
+    if (bulk) list_bulk[bb + __popcll(mb & below)] = t;
+    else list_if[bi + __popcll(mi & below)] = t;
+  }
 }
 
 // Flags of a freshly loaded state.

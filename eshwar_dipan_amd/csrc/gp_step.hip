@@ -364,6 +364,20 @@ int gp_step(gp_ctx *c, long nsteps) {
   return GP_OK;
 }
 
+int gp_sched_info(gp_ctx *c, long *ntiles, long *n_if, long *n_bulk) {
+  if (!c) return GP_E_ARG;
+  int cnt[2] = {0, 0};
+  if (c->dcnt) {
+    GP_HIP_CHECK(hipStreamSynchronize(c->stream));
+    // counters of the last step: parity before the last flip
+    GP_HIP_CHECK(hipMemcpy(cnt, c->dcnt + 2 * (c->cnt_par ^ 1), sizeof(cnt), hipMemcpyDeviceToHost));
+  }
+  if (ntiles) *ntiles = c->ntiles;
+  if (n_if) *n_if = c->path == 2 ? c->ntiles : cnt[0];
+  if (n_bulk) *n_bulk = c->path == 2 ? 0 : cnt[1];
+  return GP_OK;
+}
+
 int gp_last_step_ms(gp_ctx *c, double *ms) {
   if (!c || !ms) return GP_E_ARG;
   *ms = c->last_ms;

@@ -42,6 +42,7 @@ def lib():
             ("gp_step", [vp, l]), ("gp_stats_get", [vp, ct.POINTER(GpStats)]),
             ("gp_set_stream", [vp, vp]), ("gp_last_step_ms", [vp, ct.POINTER(d)]),
             ("gp_set_path", [vp, i]), ("gp_set_graphs", [vp, i]),
+            ("gp_sched_info", [vp, ct.POINTER(l), ct.POINTER(l), ct.POINTER(l)]),
             ("gp_read_input", [ct.c_char_p, ct.POINTER(GpInput), i, ct.c_char_p, ct.c_size_t]),
             ("gp_derive", [ct.POINTER(GpParams)]),
             ("gp_check_supported", [ct.POINTER(GpParams), ct.c_char_p, ct.c_size_t]),
@@ -175,6 +176,12 @@ class Solver:
 
     def step(self, n=1):
         _check(lib().gp_step(self._h, int(n)), "gp_step")
+
+    def sched_info(self):
+        """(tiles, interface tiles, bulk tiles) of the last fused step."""
+        a, b, c = ct.c_long(), ct.c_long(), ct.c_long()
+        _check(lib().gp_sched_info(self._h, ct.byref(a), ct.byref(b), ct.byref(c)), "gp_sched_info")
+        return a.value, b.value, c.value
 
     def last_step_ms(self):
         v = ct.c_double()

@@ -164,6 +164,10 @@ int  gp_set_path(gp_ctx *ctx, int path);
    context owns one unless gp_set_stream gave another. */
 int  gp_set_graphs(gp_ctx *ctx, int on);
 
+/* Tiles of the last fused step: total, interface (full kernel) and bulk
+   (diffusion-only kernel) counts; synchronises the context's stream. */
+int  gp_sched_info(gp_ctx *ctx, long *ntiles, long *n_interface, long *n_bulk);
+
 const char *gp_strerror(int code);
 
 #ifdef __cplusplus
