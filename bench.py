@@ -132,7 +132,8 @@ def main():
         "dtype": a.dtype,
         "data": "synthetic (seeded microstructure)",
         "config": {"workload": desc, "mesh_x": MX, "mesh_y": MY, "phases": P, "components": C + 1,
-                   "path": ["sparse interface tiles (fused LDS kernel) + bulk tiles", "two-pass", "dense fused"][a.path], "steps_to_solution": nsteps_full},
+                   "path": ["sparse: fused LDS kernel on interface tiles + bulk kernel", "two-pass", "dense fused",
+                            "sparse: two-pass kernels on interface tiles + bulk kernel"][a.path], "steps_to_solution": nsteps_full},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_cell_update": B, "kernel_ms_per_step": kern_s * 1e3},

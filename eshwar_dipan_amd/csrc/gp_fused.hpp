@@ -273,6 +273,51 @@ __global__ void __launch_bounds__(256) k_bulk(const DevParams<R>* __restrict__ p
   }
 }
 
+// Sparse two-pass engine for interface tiles: delta-phi on each listed tile
+// plus its 1-cell ring (into the dphi planes), then the update of the tile.
+template <int P, int C, class R, int TX, int TY>
+__global__ void __launch_bounds__(256) k_phase_tiles(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                                     const R* __restrict__ mu, R* __restrict__ dphi,
+                                                     const int* __restrict__ list, const int* __restrict__ count,
+                                                     int nty) {
+  const DevParams<R>& p = *prm;
+  const int ntile = *count;
+  constexpr int RX1 = TX + 2, RY1 = TY + 2;
+  for (int it = blockIdx.x; it < ntile; it += gridDim.x) {
+    const int tile = list[it];
+    const int x0 = 3 + (tile / nty) * TX - 1, y0 = 3 + (tile % nty) * TY - 1;
+    for (int i = threadIdx.x; i < RX1 * RY1; i += 256) {
+      const int x = x0 + i / RY1, y = y0 + i % RY1;
+      if (x > p.rx - 3 || y > p.ry - 3) continue;
+      phase_cell<P, C>(p, phi, mu, dphi, x, y);
+    }
+  }
+}
+
+template <int P, int C, class R, int TX, int TY>
+__global__ void __launch_bounds__(256) k_conc_tiles(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                                    const R* __restrict__ mu, const R* __restrict__ dphi,
+                                                    const R* __restrict__ mulo, R* __restrict__ phi_o,
+                                                    R* __restrict__ mu_o, R* __restrict__ mulo_o,
+                                                    const int* __restrict__ list, const int* __restrict__ count,
+                                                    int* __restrict__ flag_o, int nty) {
+  __shared__ int red[256];
+  const DevParams<R>& p = *prm;
+  const int ntile = *count;
+  for (int it = blockIdx.x; it < ntile; it += gridDim.x) {
+    const int tile = list[it];
+    const int x0 = 3 + (tile / nty) * TX, y0 = 3 + (tile % nty) * TY;
+    int f = -2;
+    for (int i = threadIdx.x; i < TX * TY; i += 256) {
+      const int x = x0 + i / TY, y = y0 + i % TY;
+      if (x > p.mesh_x + 2 || y > p.mesh_y + 2) continue;
+      f = flag_combine(f, conc_cell<P, C, R, !Ops<R>::Exact>(p, phi, mu, dphi, mulo, phi_o, mu_o, mulo_o, x, y));
+    }
+    f = block_flag(f, red);
+    if (threadIdx.x == 0) flag_o[tile] = f == -2 ? -1 : f;
+  }
+}
+
 // Tile classification for one step: a tile is bulk when it and every tile its
 // 2-cell halo reads from (deps, through the ghost copies) carry the same one-hot
 // flag.  Appends to the interface / bulk lists with atomics on cnt[0..1] and

@@ -55,13 +55,10 @@ __device__ __forceinline__ void load3x3(const DevParams<R>& p, const R* __restri
       for (int j = 0; j < 3; j++) v[a][i][j] = phi[idx(p, a, x + i - 1, y + j - 1)];
 }
 
+// delta-phi of padded cell (x,y) into dphi (calculate_divergence_phasefield_2D)
 template <int P, int C, class R>
-__global__ void __launch_bounds__(BX * BY) k_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
-                                                   const R* __restrict__ mu, R* __restrict__ dphi) {
-  const DevParams<R>& p = *prm;
-  const int y = 2 + blockIdx.x * BX + threadIdx.x;
-  const int x = 2 + blockIdx.y * BY + threadIdx.y;
-  if (x > p.rx - 3 || y > p.ry - 3) return;
+__device__ __forceinline__ void phase_cell(const DevParams<R>& p, const R* __restrict__ phi, const R* __restrict__ mu,
+                                           R* __restrict__ dphi, int x, int y) {
   R v[P][3][3];
   load3x3<P>(p, phi, x, y, v);
   Nbhd<P, R> n;
@@ -74,14 +71,13 @@ __global__ void __launch_bounds__(BX * BY) k_phase(const DevParams<R>* __restric
   for (int a = 0; a < P; a++) dphi[idx(p, a, x, y)] = dp[a];
 }
 
-template <int P, int C, class R>
-__global__ void __launch_bounds__(BX * BY) k_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
-                                                  const R* __restrict__ mu, const R* __restrict__ dphi,
-                                                  R* __restrict__ phi_o, R* __restrict__ mu_o) {
-  const DevParams<R>& p = *prm;
-  const int y = 3 + blockIdx.x * BX + threadIdx.x;
-  const int x = 3 + blockIdx.y * BY + threadIdx.y;
-  if (x > p.mesh_x + 2 || y > p.mesh_y + 2) return;
+// mu and phi update of interior cell (x,y) with its ghost images; Kahan adds
+// the mu increment to mu + mulo (binary32).  Returns the one-hot phase of the
+// updated phi (-1 if not one-hot).
+template <int P, int C, class R, bool Kahan>
+__device__ __forceinline__ int conc_cell(const DevParams<R>& p, const R* __restrict__ phi, const R* __restrict__ mu,
+                                         const R* __restrict__ dphi, const R* __restrict__ mulo, R* __restrict__ phi_o,
+                                         R* __restrict__ mu_o, R* __restrict__ mulo_o, int x, int y) {
   R v[P][3][3];
   load3x3<P>(p, phi, x, y, v);
   Nbhd<P, R> n;
@@ -142,8 +138,48 @@ __global__ void __launch_bounds__(BX * BY) k_conc(const DevParams<R>* __restrict
   cell_update<P, C>(p, phc, mc, dc, jf, jb, jr, jl, Dmf, Dmb, Dmr, Dml, gmf, gmb, gmr, gml, pho, muo, dmu);
 #pragma unroll
   for (int a = 0; a < P; a++) write_with_ghosts(p, 0, phi_o + (size_t)a * p.plane, x, y, pho[a]);
+  if constexpr (!Kahan) {
 #pragma unroll
-  for (int k = 0; k < C; k++) write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, muo[k]);
+    for (int k = 0; k < C; k++) write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, muo[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      R lo = mulo[idx(p, k, x, y)];
+      R yk = dmu[k] - lo;
+      R t = mc[k] + yk;
+      R lo_n = (t - mc[k]) - yk;
+      write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, t);
+      mulo_o[idx(p, k, x, y)] = lo_n;
+    }
+  }
+  int a1 = -1, nz = 0;
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    if (pho[a] == R(1.0)) a1 = a;
+    else if (pho[a] != R(0)) nz++;
+  }
+  return nz == 0 ? a1 : -1;
+}
+
+template <int P, int C, class R>
+__global__ void __launch_bounds__(BX * BY) k_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                                   const R* __restrict__ mu, R* __restrict__ dphi) {
+  const DevParams<R>& p = *prm;
+  const int y = 2 + blockIdx.x * BX + threadIdx.x;
+  const int x = 2 + blockIdx.y * BY + threadIdx.y;
+  if (x > p.rx - 3 || y > p.ry - 3) return;
+  phase_cell<P, C>(p, phi, mu, dphi, x, y);
+}
+
+template <int P, int C, class R>
+__global__ void __launch_bounds__(BX * BY) k_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                                  const R* __restrict__ mu, const R* __restrict__ dphi,
+                                                  R* __restrict__ phi_o, R* __restrict__ mu_o) {
+  const DevParams<R>& p = *prm;
+  const int y = 3 + blockIdx.x * BX + threadIdx.x;
+  const int x = 3 + blockIdx.y * BY + threadIdx.y;
+  if (x > p.mesh_x + 2 || y > p.mesh_y + 2) return;
+  conc_cell<P, C, R, false>(p, phi, mu, dphi, nullptr, phi_o, mu_o, nullptr, x, y);
 }
 
 // Fill the ghost layers of a freshly loaded state: every interior cell copies

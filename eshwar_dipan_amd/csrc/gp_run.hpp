@@ -91,7 +91,7 @@ inline int setup_tiles(gp_ctx *c, int TX, int TY) {
 template <int P, int C, class R>
 int run_steps(gp_ctx *c, long nsteps) {
   const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
-  if (c->path == 0 || c->path == 2) {
+  if (c->path == 0 || c->path == 2 || c->path == 3) {
     using T = FusedTile<P, C, R>;
     int rc = setup_tiles(c, T::TX, T::TY);
     if (rc) return rc;
@@ -106,17 +106,25 @@ int run_steps(gp_ctx *c, long nsteps) {
       int i = c->cur, o = c->cur ^ 1;
       int *cnt = c->dcnt + 2 * c->cnt_par, *cnt_next = c->dcnt + 2 * (c->cnt_par ^ 1);
       const int *list_if = c->dlist, *cnt_if = cnt;
-      if (c->path == 0) {
+      if (c->path != 2) {
         hipLaunchKernelGGL(k_plan<0>, dim3((nt + 255) / 256), dim3(256), 0, c->stream, (const int *)c->dflag[i],
                            (const int *)c->ddeps, c->maxdep, nt, c->dlist, c->dlist + nt, cnt, cnt_next);
       } else {
         list_if = c->dlist_all;
         cnt_if = c->dcnt + 4;
       }
-      hipLaunchKernelGGL((k_fused<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, prm,
-                         (const R *)c->phi[i], (const R *)c->mu[i], (const R *)c->mulo[i], (R *)c->phi[o],
-                         (R *)c->mu[o], (R *)c->mulo[o], list_if, cnt_if, c->dflag[o], c->nty);
-      if (c->path == 0)
+      if (c->path == 3) {
+        hipLaunchKernelGGL((k_phase_tiles<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, prm,
+                           (const R *)c->phi[i], (const R *)c->mu[i], (R *)c->dphi, list_if, cnt_if, c->nty);
+        hipLaunchKernelGGL((k_conc_tiles<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, prm,
+                           (const R *)c->phi[i], (const R *)c->mu[i], (const R *)c->dphi, (const R *)c->mulo[i],
+                           (R *)c->phi[o], (R *)c->mu[o], (R *)c->mulo[o], list_if, cnt_if, c->dflag[o], c->nty);
+      } else {
+        hipLaunchKernelGGL((k_fused<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, prm,
+                           (const R *)c->phi[i], (const R *)c->mu[i], (const R *)c->mulo[i], (R *)c->phi[o],
+                           (R *)c->mu[o], (R *)c->mulo[o], list_if, cnt_if, c->dflag[o], c->nty);
+      }
+      if (c->path != 2)
         hipLaunchKernelGGL((k_bulk<P, C, R, T::TX, T::TY>), dim3(g_bulk), dim3(256), 0, c->stream, prm,
                            (const R *)c->phi[i], (const R *)c->mu[i], (const R *)c->mulo[i], (R *)c->phi[o],
                            (R *)c->mu[o], (R *)c->mulo[o], (const int *)(c->dlist + nt), (const int *)(cnt + 1),

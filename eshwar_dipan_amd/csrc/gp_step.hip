@@ -255,7 +255,7 @@ int gp_set_graphs(gp_ctx *c, int on) {
 }
 
 int gp_set_path(gp_ctx *c, int path) {
-  if (!c || path < 0 || path > 2) return GP_E_ARG;
+  if (!c || path < 0 || path > 3) return GP_E_ARG;
   c->path = path;
   return GP_OK;
 }

@@ -1,5 +1,13 @@
 """Named synthetic workloads of the grand-potential step.
 
+Physical constants shared by the workloads (the benchmark configurations leave
+them open): parabolic free energies with A = 1, c_solid = 0.2 (+0.05 per solid
+index mod 3), c_liquid = 0.4, liquidus/solidus slopes -0.45, T_eq = 1 and
+T = 0.9 (undercooled, so seeds of radius >= 6 are supercritical), interface
+energy GAMMA = 0.5 for every pair, Tau = 0.28, epsilon = 4, dt = 0.05, D_liquid = 1,
+D_solid = 0.  With these the explicit step is stable; at GAMMA = 1 and
+T = 0.95 the 6-12 cell seeds are subcritical and melt.
+
 Each builder returns an Input.in-derived parameter set (through the library's own
 parser and constant derivation) and a seeded initial state.  The five workloads
 follow the benchmark configurations of this path:
@@ -21,7 +29,7 @@ from . import gpstep, synth
 
 
 def params(P, C, mesh, aniso=0.0, W=1, bc=(1, 1, 1, 1), solver=None, gabc=None, rot=None,
-           dt=None, eps=None, D_sol=0.0, A_off=0.0, T=None, extra=()):
+           dt=None, eps=None, D_sol=0.0, A_off=0.0, T=None, gamma=None, extra=()):
     """GpInput for a synthetic configuration, written as Input.in text and read
     back through gp_read_input + gp_derive."""
     keys = synth.base_params(P, C, mesh)
@@ -38,6 +46,8 @@ def params(P, C, mesh, aniso=0.0, W=1, bc=(1, 1, 1, 1), solver=None, gabc=None, 
         keys["epsilon"] = str(eps)
     if T is not None:
         keys["T"] = str(T)
+    if gamma is not None:
+        keys["GAMMA"] = "{" + ",".join([str(gamma)] * npair) + "}"
     solver = solver or ("BINARY" if C == 1 else "TERNARY" if C == 2 else "GENERAL")
     lines = synth.thermo_lines(P, C, solver=solver, D_sol=D_sol, A_off=A_off)
     lines += synth.bc_lines(bc, bc)
@@ -81,7 +91,7 @@ def voronoi_state(inp, npoints, seed, liquid_fraction=0.0):
 
 
 def c1(mesh=256):
-    inp = params(2, 1, (mesh, mesh))
+    inp = params(2, 1, (mesh, mesh), T=0.9, gamma=0.5)
     phi, mu = seeded_state(inp, 0, 1, 1, seed=1)
     X, Y = np.meshgrid(np.arange(mesh), np.arange(mesh), indexing="ij")
     disk = (X - mesh // 2) ** 2 + (Y - mesh // 2) ** 2 <= 10 ** 2
@@ -90,20 +100,20 @@ def c1(mesh=256):
 
 
 def c2(mesh=1024):
-    inp = params(4, 2, (mesh, mesh), aniso=0.02, bc=(3, 3, 3, 3), gabc=1.0)
+    inp = params(4, 2, (mesh, mesh), aniso=0.02, bc=(3, 3, 3, 3), gabc=1.0, T=0.9, gamma=0.5)
     phi, mu = seeded_state(inp, 32, 6, 12, seed=2, noise=1e-3)
     return inp, phi, mu, 10000, (f"3 solids + liquid, ternary, {mesh}x{mesh}, 4-fold anisotropy 0.02, "
                                  "32 seeds r 6-12, mu + N(0,1e-3), periodic")
 
 
 def c3(mesh=4096):
-    inp = params(9, 1, (mesh, mesh), gabc=0.5)
-    phi, mu = voronoi_state(inp, 200, seed=3)
-    return inp, phi, mu, 20000, f"8 solids + liquid, binary, {mesh}x{mesh}, Voronoi 200 sites"
+    inp = params(9, 1, (mesh, mesh), gabc=0.5, T=0.9, gamma=0.5)
+    phi, mu = voronoi_state(inp, 200, seed=3, liquid_fraction=0.3)
+    return inp, phi, mu, 20000, f"8 solids + liquid, binary, {mesh}x{mesh}, Voronoi grains of 200 seeded sites (30% liquid)"
 
 
 def c5(mesh=512):
-    inp = params(2, 1, (mesh, mesh), aniso=0.02)
+    inp = params(2, 1, (mesh, mesh), aniso=0.02, T=0.9, gamma=0.5)
     phi, mu = seeded_state(inp, 0, 1, 1, seed=5)
     X, Y = np.meshgrid(np.arange(mesh), np.arange(mesh), indexing="ij")
     r = np.sqrt((X - mesh / 2) ** 2 + (Y - mesh / 2) ** 2)

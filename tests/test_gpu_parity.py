@@ -17,7 +17,7 @@ def _cmp(a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("path", [1, 0, 2])
+@pytest.mark.parametrize("path", [1, 0, 2, 3])
 @pytest.mark.parametrize("name", CASES)
 def test_gpu_f64_reproduces_reference(name, path):
     print(f"\n[gpu] golden {name} path={path}", flush=True)
@@ -67,7 +67,7 @@ def test_gpu_f64_matches_oracle(name, spec, init, steps):
     print(f"\n[gpu] oracle {name} steps={steps}", flush=True)
     inp, phi0, mu0 = _oracle_case(spec, init, seed=sum(map(ord, name)))
     ophi, omu = gputil.oracle_run(inp.p, phi0, mu0, steps, nthreads=0)
-    for path in (1, 0, 2):
+    for path in (1, 0, 2, 3):
         s = gpstep.Solver(inp, "f64", path=path)
         s.set_state(phi0, mu0)
         s.step(steps)
