@@ -61,6 +61,7 @@ class GpStats(ct.Structure):
         ("phi_sum", _arr(_D, PM)),
         ("solute", _arr(_D, CM)),
         ("nonfinite", _D),
+        ("rel_change_phi", _arr(_D, PM)), ("rel_change_mu", _arr(_D, CM)),
     ]
 
 

@@ -21,6 +21,7 @@ struct gp_ctx {
   double *dpart;        // stats partials
   int nstat_blocks;
   int cur;              // index of the current state buffers
+  int stepped;          // the other buffers hold the state one step earlier
   int path;             // 0 fused, 1 two-pass
   hipStream_t stream;
   hipEvent_t ev0, ev1;

@@ -174,12 +174,13 @@ __global__ void __launch_bounds__(BX * BY) k_phase(const DevParams<R>* __restric
 template <int P, int C, class R>
 __global__ void __launch_bounds__(BX * BY) k_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                   const R* __restrict__ mu, const R* __restrict__ dphi,
-                                                  R* __restrict__ phi_o, R* __restrict__ mu_o) {
+                                                  const R* __restrict__ mulo, R* __restrict__ phi_o,
+                                                  R* __restrict__ mu_o, R* __restrict__ mulo_o) {
   const DevParams<R>& p = *prm;
   const int y = 3 + blockIdx.x * BX + threadIdx.x;
   const int x = 3 + blockIdx.y * BY + threadIdx.y;
   if (x > p.mesh_x + 2 || y > p.mesh_y + 2) return;
-  conc_cell<P, C, R, false>(p, phi, mu, dphi, nullptr, phi_o, mu_o, nullptr, x, y);
+  conc_cell<P, C, R, !Ops<R>::Exact>(p, phi, mu, dphi, mulo, phi_o, mu_o, mulo_o, x, y);
 }
 
 // Fill the ghost layers of a freshly loaded state: every interior cell copies
@@ -227,15 +228,18 @@ __global__ void k_unpack(const DevParams<R>* __restrict__ prm, const R* __restri
 // Partial layout per block: [phi_max P][phi_min P][phi_sum P][mu_max C][mu_min C][solute C][nonfinite 1]
 template <int P, int C, class R>
 __global__ void __launch_bounds__(256) k_stats(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
-                                               const R* __restrict__ mu, double* __restrict__ part) {
+                                               const R* __restrict__ mu, const R* __restrict__ phi_prev,
+                                               const R* __restrict__ mu_prev, double* __restrict__ part) {
   const DevParams<R>& p = *prm;
-  constexpr int NV = 3 * P + 3 * C + 1;
+  constexpr int NV = 3 * P + 3 * C + 1 + P + C;
   double acc[NV];
 #pragma unroll
   for (int a = 0; a < P; a++) { acc[a] = -INFINITY; acc[P + a] = INFINITY; acc[2 * P + a] = 0.0; }
 #pragma unroll
   for (int k = 0; k < C; k++) { acc[3 * P + k] = -INFINITY; acc[3 * P + C + k] = INFINITY; acc[3 * P + 2 * C + k] = 0.0; }
-  acc[NV - 1] = 0.0;
+  acc[3 * P + 3 * C] = 0.0;
+#pragma unroll
+  for (int v = 3 * P + 3 * C + 1; v < NV; v++) acc[v] = 0.0;
   const long ncell = (long)p.mesh_x * p.mesh_y;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += (long)gridDim.x * blockDim.x) {
     int x = (int)(i / p.mesh_y) + 3, y = (int)(i % p.mesh_y) + 3;
@@ -245,7 +249,11 @@ __global__ void __launch_bounds__(256) k_stats(const DevParams<R>* __restrict__ 
     for (int a = 0; a < P; a++) {
       ph[a] = phi[idx(p, a, x, y)];
       double d = (double)ph[a];
-      if (!isfinite(d)) acc[NV - 1] += 1.0;
+      if (!isfinite(d)) acc[3 * P + 3 * C] += 1.0;
+      if (phi_prev) {
+        double e = d - (double)phi_prev[idx(p, a, x, y)];
+        acc[3 * P + 3 * C + 1 + a] += e * e;
+      }
       acc[a] = fmax(acc[a], d);
       acc[P + a] = fmin(acc[P + a], d);
       acc[2 * P + a] += d;
@@ -254,7 +262,11 @@ __global__ void __launch_bounds__(256) k_stats(const DevParams<R>* __restrict__ 
     for (int k = 0; k < C; k++) {
       m[k] = mu[idx(p, k, x, y)];
       double d = (double)m[k];
-      if (!isfinite(d)) acc[NV - 1] += 1.0;
+      if (!isfinite(d)) acc[3 * P + 3 * C] += 1.0;
+      if (mu_prev) {
+        double e = d - (double)mu_prev[idx(p, k, x, y)];
+        acc[3 * P + 3 * C + 1 + P + k] += e * e;
+      }
       acc[3 * P + k] = fmax(acc[3 * P + k], d);
       acc[3 * P + C + k] = fmin(acc[3 * P + C + k], d);
     }

@@ -142,7 +142,8 @@ int run_steps(gp_ctx *c, long nsteps) {
     const R *phi = (const R *)c->phi[c->cur], *mu = (const R *)c->mu[c->cur];
     R *phi_o = (R *)c->phi[c->cur ^ 1], *mu_o = (R *)c->mu[c->cur ^ 1];
     hipLaunchKernelGGL((k_phase<P, C, R>), g_phase, blk, 0, c->stream, prm, phi, mu, (R *)c->dphi);
-    hipLaunchKernelGGL((k_conc<P, C, R>), g_conc, blk, 0, c->stream, prm, phi, mu, (const R *)c->dphi, phi_o, mu_o);
+    hipLaunchKernelGGL((k_conc<P, C, R>), g_conc, blk, 0, c->stream, prm, phi, mu, (const R *)c->dphi,
+                       (const R *)c->mulo[c->cur], phi_o, mu_o, (R *)c->mulo[c->cur ^ 1]);
     c->cur ^= 1;
   }
   c->flags_valid = 0;
@@ -152,10 +153,12 @@ int run_steps(gp_ctx *c, long nsteps) {
 
 template <int P, int C, class R>
 int run_stats(gp_ctx *c, gp_stats *st) {
-  constexpr int NV = 3 * P + 3 * C + 1;
+  constexpr int NV = 3 * P + 3 * C + 1 + P + C;
   const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
+  const R *pp = c->stepped ? (const R *)c->phi[c->cur ^ 1] : nullptr;
+  const R *mp = c->stepped ? (const R *)c->mu[c->cur ^ 1] : nullptr;
   hipLaunchKernelGGL((k_stats<P, C, R>), dim3(c->nstat_blocks), dim3(256), 0, c->stream, prm,
-                     (const R *)c->phi[c->cur], (const R *)c->mu[c->cur], c->dpart);
+                     (const R *)c->phi[c->cur], (const R *)c->mu[c->cur], pp, mp, c->dpart);
   GP_HIP_CHECK(hipGetLastError());
   std::vector<double> h((size_t)c->nstat_blocks * NV);
   GP_HIP_CHECK(hipMemcpyAsync(h.data(), c->dpart, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -175,7 +178,9 @@ int run_stats(gp_ctx *c, gp_stats *st) {
       st->mu_min[k] = fmin(st->mu_min[k], v[3 * P + C + k]);
       st->solute[k] += v[3 * P + 2 * C + k];
     }
-    st->nonfinite += v[NV - 1];
+    st->nonfinite += v[3 * P + 3 * C];
+    for (int a = 0; a < P; a++) st->rel_change_phi[a] += v[3 * P + 3 * C + 1 + a];
+    for (int k = 0; k < C; k++) st->rel_change_mu[k] += v[3 * P + 3 * C + 1 + P + k];
   }
   return GP_OK;
 }

@@ -112,6 +112,7 @@ static int load_state(gp_ctx *c, const S *dphi_src, const S *dmu_src) {
   if (c->mulo[c->cur]) GP_HIP_CHECK(hipMemsetAsync(c->mulo[c->cur], 0, (size_t)c->plane * c->esize * c->C, c->stream));
   GP_HIP_CHECK(hipGetLastError());
   c->flags_valid = 0;
+  c->stepped = 0;
   return GP_OK;
 }
 
@@ -198,7 +199,7 @@ int gp_create(const gp_params *p, int dtype, gp_ctx **out) {
   ok = ok && hipMalloc(&c->dphi, pb * c->P) == hipSuccess && hipMemset(c->dphi, 0, pb * c->P) == hipSuccess;
   long ncell = p->mesh_x * p->mesh_y;
   c->nstat_blocks = (int)std::min<long>(1024, (ncell + 255) / 256);
-  int nv = 3 * c->P + 3 * c->C + 1;
+  int nv = 3 * c->P + 3 * c->C + 1 + c->P + c->C;
   ok = ok && hipMalloc((void **)&c->dpart, sizeof(double) * c->nstat_blocks * nv) == hipSuccess;
   if (dtype == GP_F64) {
     gpk::DevParams<double> *h = new (std::nothrow) gpk::DevParams<double>();
@@ -356,6 +357,7 @@ int gp_step(gp_ctx *c, long nsteps) {
   }
   if (!rc && left > 0) rc = dispatch_steps(c, left);
   if (rc) return rc;
+  if (nsteps > 0) c->stepped = 1;
   GP_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
   GP_HIP_CHECK(hipEventSynchronize(c->ev1));
   float ms = 0;

@@ -118,6 +118,8 @@ typedef struct gp_stats {
   double phi_sum[GP_MAX_PHASES];            /* sum of phi_a over interior cells          */
   double solute[GP_MAX_COMP];               /* sum_cells w * sum_a h_a(phi) c_a(mu)       */
   double nonfinite;                         /* number of non-finite interior values      */
+  double rel_change_phi[GP_MAX_PHASES];     /* sum of (change of phi_a in the last step)^2 */
+  double rel_change_mu[GP_MAX_COMP];        /* sum of (change of mu_k in the last step)^2  */
 } gp_stats;
 
 typedef struct gp_ctx gp_ctx;

@@ -214,6 +214,46 @@ def run_case(c, ref):
         shutil.rmtree(d)
 
 
+# Driver fixtures: the reference's own output files, byte for byte, for a short
+# run in each VTK format (the drop-in driver must reproduce them exactly).
+DRIVER_CASES = [
+    dict(name="driver_p2c1_binary", P=2, C=1, mesh=(48, 40), fmt="BINARY", steps=20, saveT=10,
+         fill=["FILLCYLINDER = {0, 20, 18, 0, 0, 8};"]),
+    dict(name="driver_p3c1_ascii", P=3, C=1, mesh=(32, 36), fmt="ASCII", steps=12, saveT=6,
+         fill=["FILLCYLINDER = {0, 10, 12, 0, 0, 6};", "FILLCUBE = {1, 18, 20, 0, 28, 30, 0};"], gabc=1.0),
+]
+
+
+def run_driver_case(c, ref):
+    P, C = c["P"], c["C"]
+    keys = synth.base_params(P, C, c["mesh"])
+    keys.update({"NTIMESTEPS": str(c["steps"]), "SAVET": str(c["saveT"]), "TRACK_PROGRESS": str(c["saveT"]),
+                 "WRITEFORMAT": c["fmt"]})
+    if c.get("gabc") is not None:
+        keys["Gamma_abc"] = "{" + ",".join([str(c["gabc"])] * (P * (P - 1) * (P - 2) // 6)) + "}"
+    infile = synth.infile_text(keys, synth.thermo_lines(P, C) + synth.bc_lines())
+    filling = "\n".join(c["fill"]) + "\n"
+    d = tempfile.mkdtemp(prefix="gpdrv_")
+    try:
+        open(os.path.join(d, "Input.in"), "w").write(infile)
+        open(os.path.join(d, "Filling.in"), "w").write(filling)
+        r = subprocess.run([ref, "Input.in", "Filling.in", "out"], cwd=d, capture_output=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"{c['name']}: reference exited {r.returncode}")
+        res = {"infile": np.array(infile), "filling": np.array(filling)}
+        for echo in ("Input.out", "Input.bd"):
+            if os.path.exists(os.path.join(d, echo)):
+                res["echo_" + echo.replace(".", "_")] = np.frombuffer(open(os.path.join(d, echo), "rb").read(), dtype=np.uint8)
+        res["stdout"] = np.frombuffer(r.stdout, dtype=np.uint8)
+        files = sorted(os.listdir(os.path.join(d, "DATA")))
+        res["files"] = np.array(files)
+        for f in files:
+            res["file_" + f] = np.frombuffer(open(os.path.join(d, "DATA", f), "rb").read(), dtype=np.uint8)
+        return res
+    finally:
+        shutil.rmtree(d)
+
+
 def main():
     for r in REF:
         if not os.path.exists(r):
@@ -234,6 +274,16 @@ def main():
         np.savez_compressed(os.path.join(OUT, c["name"] + ".npz"), **a)
         print(f"{c['name']}: P={c['P']} C={c['C']} mesh={c['mesh']} steps={c['save']} "
               f"max|dphi|={moved:.3e} -O0/-O2 identical")
+    for c in DRIVER_CASES:
+        if only and c["name"] not in only:
+            continue
+        a = run_driver_case(c, REF[0])
+        b = run_driver_case(c, REF[1])
+        for k in a:
+            if k.startswith("file_") and not np.array_equal(a[k], b[k]):
+                sys.exit(f"{c['name']}: -O0 and -O2 reference builds differ in {k}")
+        np.savez_compressed(os.path.join(OUT, c["name"] + ".npz"), **a)
+        print(f"{c['name']}: files {list(a['files'])}")
 
 
 if __name__ == "__main__":

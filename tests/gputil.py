@@ -62,7 +62,14 @@ def oracle_dphi(p, phi, mu):
 
 
 def golden_cases():
-    return sorted(os.path.basename(f)[:-4] for f in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    """Step fixtures (state before/after n steps of the reference)."""
+    return sorted(os.path.basename(f)[:-4] for f in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                  if not os.path.basename(f).startswith("driver_"))
+
+
+def driver_cases():
+    """Driver fixtures (the reference's own output files of a short run)."""
+    return sorted(os.path.basename(f)[:-4] for f in glob.glob(os.path.join(GOLDEN, "driver_*.npz")))
 
 
 def load_golden(name):
