@@ -56,13 +56,26 @@ def cpu_baseline(inp, phi0, mu0, steps, threads):
     return MX * MY * steps / dt, dt, steps
 
 
+def precision_report(a):
+    from eshwar_dipan_amd import precision, workloads
+    inp, phi0, mu0, nsteps_full, desc = workloads.WORKLOADS[a.workload or "c3"]()
+    steps = a.steps or 2000
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    r = precision.compare(inp, phi0, mu0, steps, chunk=500, path=a.path, log=log)
+    r.update({"metric": "binary32 vs binary64: per-field error, solute drift, time-to-solution ratio",
+              "config": {"workload": desc, "mesh_x": inp.p.mesh_x, "mesh_y": inp.p.mesh_y,
+                         "phases": inp.p.nphases, "components": inp.p.ncomp + 1, "path": a.path,
+                         "steps_to_solution": nsteps_full}})
+    print(json.dumps(r), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default: the workload's steps to solution, capped at 2e10 cell updates)")
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default=None, help="c1, c2 (default), c3, c5")
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--path", type=int, default=0, help="0 sparse interface scheduling, 1 two-pass, 2 dense fused")
     ap.add_argument("--no-graphs", action="store_true")
@@ -70,7 +83,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy five times (calibration of the HBM byte counters)")
+    ap.add_argument("--report", choices=["precision"], default=None,
+                    help="precision: binary32 against binary64 on --workload (default c3), per-field error, "
+                         "solute drift and time-to-solution ratio over --steps (default 2000)")
     a = ap.parse_args()
+    if a.report == "precision":
+        precision_report(a)
+        return
     if a.copy_probe:
         import torch
         n = 1 << 28
@@ -83,11 +102,12 @@ def main():
         return
     if a.gpus != 1:
         sys.exit(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not run "
-                 "(the step shards by x-strips with 3-row halos; see DESIGN.md)")
+                 "(the step would shard by x-strips with 3-row halos, one rank per GPU)")
 
     import torch
     from eshwar_dipan_amd import gpstep, workloads
 
+    a.workload = a.workload or "c2"
     inp, phi0, mu0, nsteps_full, desc = workloads.WORKLOADS[a.workload]()
     p = inp.p
     P, C, MX, MY = p.nphases, p.ncomp, p.mesh_x, p.mesh_y

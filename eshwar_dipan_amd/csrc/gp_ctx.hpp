@@ -17,6 +17,7 @@ struct gp_ctx {
   void *phi[2], *mu[2]; // ping-pong field buffers (padded planes)
   void *mulo[2];        // binary32 path: low-order part of mu (compensated update)
   void *dphi;           // two-pass path: delta-phi planes
+  signed char *dcode;   // sparse two-pass path: per-cell one-hot / zero-delta-phi code
   void *dprm;           // DevParams<R> on the device
   double *dpart;        // stats partials
   int nstat_blocks;

@@ -58,7 +58,7 @@ __device__ __forceinline__ int block_flag(int f, int* red) {
 }
 
 template <int P, int C, class R, int TX, int TY>
-__global__ void __launch_bounds__(256) k_fused(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+__global__ void __launch_bounds__(256) GP_HEAVY k_fused(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                const R* __restrict__ mu, const R* __restrict__ mulo,
                                                R* __restrict__ phi_o, R* __restrict__ mu_o, R* __restrict__ mulo_o,
                                                const int* __restrict__ list, const int* __restrict__ count,
@@ -203,11 +203,63 @@ __global__ void __launch_bounds__(256) k_fused(const DevParams<R>* __restrict__ 
   }  // tile loop
 }
 
+// mu/phi update of a cell whose 3x3 block is one-hot in one phase and whose own
+// and four neighbours' delta-phi are +0: the antitrapping fluxes and the dh/dt
+// source are exact zeros and the five mobilities are equal, so the update is the
+// bulk branch of cell_update with those zeros -- bit-identical to conc_cell.
+template <int P, int C, class R>
+__device__ __forceinline__ void bulk_cell(const DevParams<R>& p, const R* __restrict__ phi, const R* __restrict__ mu,
+                                          const R* __restrict__ mulo, R* __restrict__ phi_o, R* __restrict__ mu_o,
+                                          R* __restrict__ mulo_o, int x, int y) {
+  R ph[P], zp[P], mc[C], mf[C], mb[C], mr[C], ml[C], z[C];
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    ph[a] = phi[idx(p, a, x, y)];
+    zp[a] = R(0);
+  }
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    mc[k] = mu[idx(p, k, x, y)];
+    mf[k] = mu[idx(p, k, x + 1, y)];
+    mb[k] = mu[idx(p, k, x - 1, y)];
+    mr[k] = mu[idx(p, k, x, y + 1)];
+    ml[k] = mu[idx(p, k, x, y - 1)];
+    z[k] = R(0);
+  }
+  // the five cells carry the same phi, hence the same mobility
+  R D[C][C], Dm[C][C], gmf[C], gmb[C], gmr[C], gml[C];
+  cell_D<P, C>(p, ph, D);
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+#pragma unroll
+    for (int l = 0; l < C; l++) Dm[k][l] = (D[k][l] + D[k][l]) * R(0.5);
+    gmf[k] = div_dx(mf[k] - mc[k], p);
+    gmb[k] = div_dx(mc[k] - mb[k], p);
+    gmr[k] = div_dy(mr[k] - mc[k], p);
+    gml[k] = div_dy(mc[k] - ml[k], p);
+  }
+  R pho[P], muo[C], dmu[C];
+  cell_update<P, C>(p, ph, mc, zp, z, z, z, z, Dm, Dm, Dm, Dm, gmf, gmb, gmr, gml, pho, muo, dmu);
+#pragma unroll
+  for (int a = 0; a < P; a++) write_with_ghosts(p, 0, phi_o + (size_t)a * p.plane, x, y, pho[a]);
+  if constexpr (Ops<R>::Exact) {
+#pragma unroll
+    for (int k = 0; k < C; k++) write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, muo[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      R lo = mulo[idx(p, k, x, y)];
+      R yk = dmu[k] - lo;
+      R t = mc[k] + yk;
+      R lo_n = (t - mc[k]) - yk;
+      write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, t);
+      mulo_o[idx(p, k, x, y)] = lo_n;
+    }
+  }
+}
+
 // Bulk tiles: the tile and its 2-cell halo are one-hot in one phase, so every
-// delta-phi and antitrapping flux the full step would compute for these cells is
-// an exact zero and every cell takes the bulk branch of the mu update with the
-// same mobility.  The update below passes those zeros through the same
-// cell_update, so the results equal the full kernel's bit for bit; phi is copied.
+// cell satisfies the bulk_cell conditions; phi is copied.
 template <int P, int C, class R, int TX, int TY>
 __global__ void __launch_bounds__(256) k_bulk(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                               const R* __restrict__ mu, const R* __restrict__ mulo,
@@ -223,95 +275,134 @@ __global__ void __launch_bounds__(256) k_bulk(const DevParams<R>* __restrict__ p
     for (int i = threadIdx.x; i < TX * TY; i += 256) {
       const int x = x0 + i / TY, y = y0 + i % TY;
       if (x > p.mesh_x + 2 || y > p.mesh_y + 2) continue;
-      R ph[P], zp[P], mc[C], mf[C], mb[C], mr[C], ml[C], z[C];
-#pragma unroll
-      for (int a = 0; a < P; a++) {
-        ph[a] = phi[idx(p, a, x, y)];
-        zp[a] = R(0);
-      }
-#pragma unroll
-      for (int k = 0; k < C; k++) {
-        mc[k] = mu[idx(p, k, x, y)];
-        mf[k] = mu[idx(p, k, x + 1, y)];
-        mb[k] = mu[idx(p, k, x - 1, y)];
-        mr[k] = mu[idx(p, k, x, y + 1)];
-        ml[k] = mu[idx(p, k, x, y - 1)];
-        z[k] = R(0);
-      }
-      // the five cells carry the same phi, hence the same mobility
-      R D[C][C], Dm[C][C], gmf[C], gmb[C], gmr[C], gml[C];
-      cell_D<P, C>(p, ph, D);
-#pragma unroll
-      for (int k = 0; k < C; k++) {
-#pragma unroll
-        for (int l = 0; l < C; l++) Dm[k][l] = (D[k][l] + D[k][l]) * R(0.5);
-        gmf[k] = div_dx(mf[k] - mc[k], p);
-        gmb[k] = div_dx(mc[k] - mb[k], p);
-        gmr[k] = div_dy(mr[k] - mc[k], p);
-        gml[k] = div_dy(mc[k] - ml[k], p);
-      }
-      R pho[P], muo[C], dmu[C];
-      cell_update<P, C>(p, ph, mc, zp, z, z, z, z, Dm, Dm, Dm, Dm, gmf, gmb, gmr, gml, pho, muo, dmu);
-#pragma unroll
-      for (int a = 0; a < P; a++) write_with_ghosts(p, 0, phi_o + (size_t)a * p.plane, x, y, pho[a]);
-      if constexpr (Ops<R>::Exact) {
-#pragma unroll
-        for (int k = 0; k < C; k++) write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, muo[k]);
-      } else {
-#pragma unroll
-        for (int k = 0; k < C; k++) {
-          R lo = mulo[idx(p, k, x, y)];
-          R yk = dmu[k] - lo;
-          R t = mc[k] + yk;
-          R lo_n = (t - mc[k]) - yk;
-          write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, t);
-          mulo_o[idx(p, k, x, y)] = lo_n;
-        }
-      }
+      bulk_cell<P, C, R>(p, phi, mu, mulo, phi_o, mu_o, mulo_o, x, y);
     }
     if (threadIdx.x == 0) flag_o[tile] = flag_i[tile];
   }
 }
 
-// Sparse two-pass engine for interface tiles: delta-phi on each listed tile
-// plus its 1-cell ring (into the dphi planes), then the update of the tile.
+// Interface-cell test of the phase sweep: some phase has a non-zero divergence
+// (the activity test of cell_dphi, same expressions).  Cells that fail it get
+// delta-phi from the projection alone.
+template <int P, class R>
+__device__ __forceinline__ bool phase_active(const DevParams<R>& p, const R* __restrict__ phi, int x, int y, R* ph) {
+  bool any = false;
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    const R c = phi[idx(p, a, x, y)];
+    const R gxc = div_dx(phi[idx(p, a, x + 1, y)] - c, p), gxb = div_dx(c - phi[idx(p, a, x - 1, y)], p);
+    const R gyc = div_dy(phi[idx(p, a, x, y + 1)] - c, p), gyl = div_dy(c - phi[idx(p, a, x, y - 1)], p);
+    R d = div_dx(gxc - gxb, p);
+    d += div_dy(gyc - gyl, p);
+    any = any || fabs(d) > R(0);
+    ph[a] = c;
+  }
+  return any;
+}
+
+// Sparse two-pass engine for interface tiles.  k_phase_tiles computes
+// delta-phi on each listed tile plus its 1-cell ring into the dphi planes;
+// k_conc_tiles then updates the tile.  Both kernels first sort the tile's cells
+// in LDS into a cheap class (phase sweep: no active phase, projection only;
+// concentration sweep: bulk_cell) handled at once and a list of full cells that
+// the block's threads then share, so the expensive interface arithmetic runs on
+// densely packed waves.
+//
+// The phase sweep also writes a one-byte code per cell: the phase a if phi is
+// one-hot in a and delta-phi is +0 in every phase, else -1.  A cell whose 3x3
+// block carries one code a >= 0 meets the bulk_cell conditions.
 template <int P, int C, class R, int TX, int TY>
-__global__ void __launch_bounds__(256) k_phase_tiles(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+__global__ void __launch_bounds__(256) GP_HEAVY k_phase_tiles(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                      const R* __restrict__ mu, R* __restrict__ dphi,
+                                                     signed char* __restrict__ code,
                                                      const int* __restrict__ list, const int* __restrict__ count,
                                                      int nty) {
+  constexpr int RX1 = TX + 2, RY1 = TY + 2;
+  __shared__ short todo[RX1 * RY1];
+  __shared__ int ntodo;
   const DevParams<R>& p = *prm;
   const int ntile = *count;
-  constexpr int RX1 = TX + 2, RY1 = TY + 2;
   for (int it = blockIdx.x; it < ntile; it += gridDim.x) {
     const int tile = list[it];
     const int x0 = 3 + (tile / nty) * TX - 1, y0 = 3 + (tile % nty) * TY - 1;
+    if (threadIdx.x == 0) ntodo = 0;
+    __syncthreads();
     for (int i = threadIdx.x; i < RX1 * RY1; i += 256) {
       const int x = x0 + i / RY1, y = y0 + i % RY1;
       if (x > p.rx - 3 || y > p.ry - 3) continue;
-      phase_cell<P, C>(p, phi, mu, dphi, x, y);
+      R ph[P];
+      if (phase_active<P>(p, phi, x, y, ph)) {
+        todo[atomicAdd(&ntodo, 1)] = (short)i;
+      } else {
+        R dp[P];
+        bool act[P];
+#pragma unroll
+        for (int a = 0; a < P; a++) {
+          dp[a] = R(0);
+          act[a] = false;
+        }
+        project_simplex<P>(ph, act, dp);
+        bool zero = true;
+#pragma unroll
+        for (int a = 0; a < P; a++) {
+          dphi[idx(p, a, x, y)] = dp[a];
+          zero = zero && dp[a] == R(0) && !signbit(dp[a]);
+        }
+        code[(size_t)x * p.pitch + y] = (signed char)(zero ? onehot_phase<P>(ph) : -1);
+        continue;
+      }
+      code[(size_t)x * p.pitch + y] = -1;
     }
+    __syncthreads();
+    const int n = ntodo;
+    for (int j = threadIdx.x; j < n; j += 256) {
+      const int i = todo[j];
+      phase_cell<P, C>(p, phi, mu, dphi, x0 + i / RY1, y0 + i % RY1);
+    }
+    __syncthreads();
   }
 }
 
 template <int P, int C, class R, int TX, int TY>
-__global__ void __launch_bounds__(256) k_conc_tiles(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+__global__ void __launch_bounds__(256) GP_HEAVY k_conc_tiles(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                     const R* __restrict__ mu, const R* __restrict__ dphi,
+                                                    const signed char* __restrict__ code,
                                                     const R* __restrict__ mulo, R* __restrict__ phi_o,
                                                     R* __restrict__ mu_o, R* __restrict__ mulo_o,
                                                     const int* __restrict__ list, const int* __restrict__ count,
                                                     int* __restrict__ flag_o, int nty) {
   __shared__ int red[256];
+  __shared__ short todo[TX * TY];
+  __shared__ int ntodo;
   const DevParams<R>& p = *prm;
   const int ntile = *count;
   for (int it = blockIdx.x; it < ntile; it += gridDim.x) {
     const int tile = list[it];
     const int x0 = 3 + (tile / nty) * TX, y0 = 3 + (tile % nty) * TY;
+    if (threadIdx.x == 0) ntodo = 0;
+    __syncthreads();
     int f = -2;
     for (int i = threadIdx.x; i < TX * TY; i += 256) {
       const int x = x0 + i / TY, y = y0 + i % TY;
       if (x > p.mesh_x + 2 || y > p.mesh_y + 2) continue;
-      f = flag_combine(f, conc_cell<P, C, R, !Ops<R>::Exact>(p, phi, mu, dphi, mulo, phi_o, mu_o, mulo_o, x, y));
+      int cc = code[(size_t)x * p.pitch + y];
+#pragma unroll
+      for (int u = -1; u <= 1; u++)
+#pragma unroll
+        for (int w = -1; w <= 1; w++) cc = code[(size_t)(x + u) * p.pitch + y + w] == cc ? cc : -1;
+      if (cc >= 0) {
+        bulk_cell<P, C, R>(p, phi, mu, mulo, phi_o, mu_o, mulo_o, x, y);
+        f = flag_combine(f, cc);
+      } else {
+        todo[atomicAdd(&ntodo, 1)] = (short)i;
+      }
+    }
+    __syncthreads();
+    const int n = ntodo;
+    for (int j = threadIdx.x; j < n; j += 256) {
+      const int i = todo[j];
+      f = flag_combine(f, conc_cell<P, C, R, !Ops<R>::Exact>(p, phi, mu, dphi, mulo, phi_o, mu_o, mulo_o,
+                                                             x0 + i / TY, y0 + i % TY));
     }
     f = block_flag(f, red);
     if (threadIdx.x == 0) flag_o[tile] = f == -2 ? -1 : f;

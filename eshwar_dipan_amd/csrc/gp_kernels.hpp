@@ -15,6 +15,15 @@
 
 #include "gp_cell.hpp"
 
+// Waves per SIMD that the interface kernels are compiled for: their cell
+// arithmetic holds a few hundred values live, and without a bound the compiler
+// spends the whole register file on one wave per SIMD, which leaves the FP64
+// and memory latencies unhidden.
+#ifndef GP_WAVES_PER_EU
+#define GP_WAVES_PER_EU 2
+#endif
+#define GP_HEAVY __attribute__((amdgpu_waves_per_eu(GP_WAVES_PER_EU)))
+
 namespace gpk {
 
 constexpr int BX = 64;  // threads along y (contiguous)
@@ -162,7 +171,7 @@ __device__ __forceinline__ int conc_cell(const DevParams<R>& p, const R* __restr
 }
 
 template <int P, int C, class R>
-__global__ void __launch_bounds__(BX * BY) k_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+__global__ void __launch_bounds__(BX * BY) GP_HEAVY k_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                    const R* __restrict__ mu, R* __restrict__ dphi) {
   const DevParams<R>& p = *prm;
   const int y = 2 + blockIdx.x * BX + threadIdx.x;
@@ -172,7 +181,7 @@ __global__ void __launch_bounds__(BX * BY) k_phase(const DevParams<R>* __restric
 }
 
 template <int P, int C, class R>
-__global__ void __launch_bounds__(BX * BY) k_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+__global__ void __launch_bounds__(BX * BY) GP_HEAVY k_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                   const R* __restrict__ mu, const R* __restrict__ dphi,
                                                   const R* __restrict__ mulo, R* __restrict__ phi_o,
                                                   R* __restrict__ mu_o, R* __restrict__ mulo_o) {

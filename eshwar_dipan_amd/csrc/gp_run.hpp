@@ -115,9 +115,11 @@ int run_steps(gp_ctx *c, long nsteps) {
       }
       if (c->path == 3) {
         hipLaunchKernelGGL((k_phase_tiles<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, prm,
-                           (const R *)c->phi[i], (const R *)c->mu[i], (R *)c->dphi, list_if, cnt_if, c->nty);
+                           (const R *)c->phi[i], (const R *)c->mu[i], (R *)c->dphi, c->dcode, list_if, cnt_if,
+                           c->nty);
         hipLaunchKernelGGL((k_conc_tiles<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, prm,
-                           (const R *)c->phi[i], (const R *)c->mu[i], (const R *)c->dphi, (const R *)c->mulo[i],
+                           (const R *)c->phi[i], (const R *)c->mu[i], (const R *)c->dphi,
+                           (const signed char *)c->dcode, (const R *)c->mulo[i],
                            (R *)c->phi[o], (R *)c->mu[o], (R *)c->mulo[o], list_if, cnt_if, c->dflag[o], c->nty);
       } else {
         hipLaunchKernelGGL((k_fused<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, prm,

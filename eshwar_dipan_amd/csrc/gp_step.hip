@@ -150,6 +150,7 @@ static void free_ctx(gp_ctx *c) {
     if (c->mulo[i]) (void)hipFree(c->mulo[i]);
   }
   if (c->dphi) (void)hipFree(c->dphi);
+  if (c->dcode) (void)hipFree(c->dcode);
   if (c->dprm) (void)hipFree(c->dprm);
   if (c->dpart) (void)hipFree(c->dpart);
   for (int i = 0; i < 2; i++)
@@ -197,6 +198,8 @@ int gp_create(const gp_params *p, int dtype, gp_ctx **out) {
                  (dtype == GP_F64 || hipMemset(c->mulo[i], 0, pb * c->C) == hipSuccess);
   }
   ok = ok && hipMalloc(&c->dphi, pb * c->P) == hipSuccess && hipMemset(c->dphi, 0, pb * c->P) == hipSuccess;
+  ok = ok && hipMalloc((void **)&c->dcode, (size_t)c->plane) == hipSuccess &&
+       hipMemset(c->dcode, 0xff, (size_t)c->plane) == hipSuccess;
   long ncell = p->mesh_x * p->mesh_y;
   c->nstat_blocks = (int)std::min<long>(1024, (ncell + 255) / 256);
   int nv = 3 * c->P + 3 * c->C + 1 + c->P + c->C;
@@ -384,6 +387,11 @@ int gp_last_step_ms(gp_ctx *c, double *ms) {
   if (!c || !ms) return GP_E_ARG;
   *ms = c->last_ms;
   return GP_OK;
+}
+
+int gp_synchronize(gp_ctx *c) {
+  if (!c) return GP_E_ARG;
+  return hipStreamSynchronize(c->stream) == hipSuccess ? GP_OK : GP_E_HIP;
 }
 
 int gp_stats_get(gp_ctx *c, gp_stats *st) {

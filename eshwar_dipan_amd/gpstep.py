@@ -39,7 +39,7 @@ def lib():
             ("gp_create", [ct.POINTER(GpParams), i, ct.POINTER(vp)]),
             ("gp_set_state", [vp, vp, vp]), ("gp_get_state", [vp, vp, vp]),
             ("gp_set_state_device", [vp, vp, vp]), ("gp_get_state_device", [vp, vp, vp]),
-            ("gp_step", [vp, l]), ("gp_stats_get", [vp, ct.POINTER(GpStats)]),
+            ("gp_step", [vp, l]), ("gp_synchronize", [vp]), ("gp_stats_get", [vp, ct.POINTER(GpStats)]),
             ("gp_set_stream", [vp, vp]), ("gp_last_step_ms", [vp, ct.POINTER(d)]),
             ("gp_set_path", [vp, i]), ("gp_set_graphs", [vp, i]),
             ("gp_sched_info", [vp, ct.POINTER(l), ct.POINTER(l), ct.POINTER(l)]),
@@ -177,6 +177,9 @@ class Solver:
     def step(self, n=1):
         _check(lib().gp_step(self._h, int(n)), "gp_step")
 
+    def synchronize(self):
+        _check(lib().gp_synchronize(self._h), "gp_synchronize")
+
     def sched_info(self):
         """(tiles, interface tiles, bulk tiles) of the last fused step."""
         a, b, c = ct.c_long(), ct.c_long(), ct.c_long()
@@ -199,4 +202,5 @@ class Solver:
             "phi_sum": list(st.phi_sum[:P]), "mu_max": list(st.mu_max[:C]),
             "mu_min": list(st.mu_min[:C]), "solute": list(st.solute[:C]),
             "nonfinite": st.nonfinite,
+            "rel_change_phi": list(st.rel_change_phi[:P]), "rel_change_mu": list(st.rel_change_mu[:C]),
         }

@@ -1,0 +1,64 @@
+"""binary32 against binary64: per-field error and solute drift of one workload.
+
+Both contexts start from the same binary64 state and run the same number of
+steps; errors are taken on the interior fields read back after the last step,
+and solute totals come from the library's binary64 reduction
+(sum over cells of w * sum_a h_a(phi) c_a(mu), half weights on boundary nodes).
+"""
+import time
+
+import numpy as np
+
+from . import gpstep
+
+
+def _run(inp, phi0, mu0, dtype, steps, chunk, path, log):
+    s = gpstep.Solver(inp, dtype, path=path)
+    s.set_state(phi0, mu0)
+    sol0 = np.array(s.stats()["solute"])
+    s.step(min(chunk, steps))  # warm-up (graph capture), then restart
+    s.set_state(phi0, mu0)
+    s.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        s.step(n)
+        done += n
+        if log:
+            s.synchronize()
+            log(f"  {dtype}: {done}/{steps} steps, {time.perf_counter() - t0:.1f} s")
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    st = s.stats()
+    phi, mu = s.get_state()
+    s.close()
+    return phi, mu, sol0, np.array(st["solute"]), wall
+
+
+def compare(inp, phi0, mu0, steps, chunk=500, path=0, log=None):
+    """Run binary64 and binary32 for `steps` steps; returns a dict of errors,
+    solute drifts and the time-to-solution ratio."""
+    p64, m64, s0_64, s1_64, w64 = _run(inp, phi0, mu0, "f64", steps, chunk, path, log)
+    p32, m32, s0_32, s1_32, w32 = _run(inp, phi0, mu0, "f32", steps, chunk, path, log)
+    P, C = p64.shape[0], m64.shape[0]
+    out = {"steps": steps, "time_f64_s": w64, "time_f32_s": w32, "speedup_f32": w64 / w32, "phi": [], "mu": []}
+    for a in range(P):
+        d = p32[a] - p64[a]
+        out["phi"].append({"max_abs_err": float(np.abs(d).max()),
+                           "rel_l2_err": float(np.linalg.norm(d) / max(np.linalg.norm(p64[a]), 1e-300))})
+    for k in range(C):
+        d = m32[k] - m64[k]
+        scale = float(np.abs(m64[k]).max())
+        out["mu"].append({"max_abs_err": float(np.abs(d).max()), "max_rel_err": float(np.abs(d).max() / scale),
+                          "rel_l2_err": float(np.linalg.norm(d) / max(np.linalg.norm(m64[k]), 1e-300))})
+    out["solute"] = [{"f64_drift": float((s1_64[k] - s0_64[k]) / s0_64[k]),
+                      "f32_drift": float((s1_32[k] - s0_32[k]) / s0_32[k]),
+                      "f32_minus_f64": float((s1_32[k] - s1_64[k]) / s0_64[k]),
+                      "initial_f32_minus_f64": float((s0_32[k] - s0_64[k]) / s0_64[k])} for k in range(C)]
+    out["phi_max_abs_err"] = max(e["max_abs_err"] for e in out["phi"])
+    out["phi_cells_above_1e-4"] = int((np.abs(p32 - p64).max(axis=0) > 1e-4).sum())
+    out["solute_drift_f32_minus_f64_max"] = max(abs(e["f32_drift"] - e["f64_drift"]) for e in out["solute"])
+    out["mu_max_rel_err"] = max(e["max_rel_err"] for e in out["mu"])
+    out["solute_f32_minus_f64_max"] = max(abs(e["f32_minus_f64"]) for e in out["solute"])
+    return out

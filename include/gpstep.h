@@ -145,6 +145,9 @@ int  gp_get_state_device(gp_ctx *ctx, void *phi, void *mu);
    concentration sweep and ghost copy of microsim_gp.c:103-106. */
 int  gp_step(gp_ctx *ctx, long nsteps);
 
+/* Wait for all work of the context (host-side timing without another runtime). */
+int  gp_synchronize(gp_ctx *ctx);
+
 /* Diagnostics of the current state. */
 int  gp_stats_get(gp_ctx *ctx, gp_stats *st);
 
