@@ -69,16 +69,81 @@ def precision_report(a):
     print(json.dumps(r), flush=True)
 
 
+def bench_ensemble(a):
+    """Ensemble workload: every member grid stepped together as one tile set."""
+    import torch
+    import numpy as np
+    from eshwar_dipan_amd import gpstep, workloads
+    members, nsteps_full, desc = workloads.c4()
+    if a.steps is None:
+        a.steps = nsteps_full
+    e = gpstep.Ensemble([m[0] for m in members], a.dtype)
+    for i, (_, phi, mu) in enumerate(members):
+        e.set_state(i, phi, mu)
+    tile = None
+    if not a.no_autotune:
+        tile = e.autotune(32)
+    if a.warmup:
+        e.step(a.warmup)
+    for i, (_, phi, mu) in enumerate(members):  # the timed steps start from the initial states
+        e.set_state(i, phi, mu)
+    e.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.step(a.steps)
+    e.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = e.cells()
+    kern_s = e.last_step_ms() * 1e-3 / a.steps
+    esize = 8 if a.dtype == "f64" else 4
+    B = 2 * (2 + 1) * esize
+    achieved = B * cells / kern_s / 1e9
+    nt, nif, nb, tx, ty = e.sched_info()
+    nonfinite = sum(e.stats(i, allow_nonfinite=True)["nonfinite"] for i in range(len(members)))
+    out = {
+        "metric": "cell updates/s + % HBM roofline, FP32 vs FP64 at stated err; adjoint/fwd time",
+        "value": cells * a.steps / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": a.dtype, "data": "synthetic (seeded microstructure)",
+        "config": {"workload": desc, "members": len(members), "cells": cells, "phases": 2, "components": 2,
+                   "tile": [tx, ty], "autotuned": tile is not None, "steps_to_solution": nsteps_full},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_update": B,
+                     "kernel_ms_per_step": kern_s * 1e3},
+        "nonfinite": nonfinite,
+        "tiles_last_step": {"total": nt, "interface": nif, "bulk": nb},
+    }
+    if not a.no_cpu:
+        # the oracle on whole members in turn until about 10 s of work
+        threads = min(16, os.cpu_count() or 1)
+        done_cells, done_s, used = 0, 0.0, 0
+        for inp, phi, mu in members:
+            csteps = max(2, min(a.steps, int(2e8 // (inp.p.mesh_x * inp.p.mesh_y))))
+            v, dt, cs = cpu_baseline(inp, phi, mu, csteps, threads)
+            done_cells += inp.p.mesh_x * inp.p.mesh_y * cs
+            done_s += dt
+            used += 1
+            if done_s > 10.0:
+                break
+        out["cpu_baseline"] = {"value": done_cells / done_s, "unit": "cell-updates/s", "cores": threads,
+                               "kind": "port", "sample": f"first {used} members of the ensemble, each stepped "
+                                                         f"by the binary64 oracle ({done_s:.1f} s, OpenMP {threads} threads)"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default: the workload's steps to solution, capped at 2e10 cell updates)")
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default=None, help="c1, c2 (default), c3, c5")
+    ap.add_argument("--workload", default=None, help="c1, c2 (default), c3, c4 (ensemble), c5")
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
-    ap.add_argument("--path", type=int, default=0, help="0 sparse interface scheduling, 1 two-pass, 2 dense fused")
+    ap.add_argument("--path", type=int, default=0,
+                    help="0 sparse fused, 1 two-pass, 2 dense fused, 3 sparse two-pass, -1 autotune")
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="ensemble: keep the default tile shape")
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -104,6 +169,9 @@ def main():
         sys.exit(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not run "
                  "(the step would shard by x-strips with 3-row halos, one rank per GPU)")
 
+    if a.workload == "c4":
+        bench_ensemble(a)
+        return
     import torch
     from eshwar_dipan_amd import gpstep, workloads
 
@@ -113,8 +181,11 @@ def main():
     P, C, MX, MY = p.nphases, p.ncomp, p.mesh_x, p.mesh_y
     if a.steps is None:
         a.steps = int(min(nsteps_full, max(1, 2e10 // (MX * MY))))
-    s = gpstep.Solver(inp, a.dtype, path=a.path, graphs=not a.no_graphs)
+    s = gpstep.Solver(inp, a.dtype, path=a.path if a.path >= 0 else 0, graphs=not a.no_graphs)
     s.set_state(phi0, mu0)
+    autotuned = a.path < 0
+    if autotuned:
+        a.path, _ = s.autotune(64)
     s.step(a.warmup)
     s.set_state(phi0, mu0)  # the timed steps start from the initial state
     torch.cuda.synchronize()
@@ -153,7 +224,9 @@ def main():
         "data": "synthetic (seeded microstructure)",
         "config": {"workload": desc, "mesh_x": MX, "mesh_y": MY, "phases": P, "components": C + 1,
                    "path": ["sparse: fused LDS kernel on interface tiles + bulk kernel", "two-pass", "dense fused",
-                            "sparse: two-pass kernels on interface tiles + bulk kernel"][a.path], "steps_to_solution": nsteps_full},
+                            "sparse: two-pass kernels on interface tiles + bulk kernel"][a.path],
+                   "tile": list(s.tile_shape()) if a.path != 1 else None, "autotuned": autotuned,
+                   "steps_to_solution": nsteps_full},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_cell_update": B, "kernel_ms_per_step": kern_s * 1e3},

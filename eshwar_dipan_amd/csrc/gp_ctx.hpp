@@ -43,11 +43,24 @@ struct gp_ctx {
   hipGraphExec_t graph[2];
   int graph_path;
   int use_graphs;
+  // grids of the tile set: a context steps itself; an ensemble master (nmember
+  // > 0) owns no fields and steps its member contexts as one tile set
+  int nmember;
+  gp_ctx **member;
+  int tile0;             // first global tile of this grid in its tile set
+  void *dinst;           // device Inst<R>[max(1, nmember)]
+  int *dtinst;           // ensemble: global tile -> member (null for one grid)
+  int shape;             // tile shape of the sparse two-pass path (index into the candidate list)
+  int shape_set;         // tile shape the tile set was built for (-1: none)
 };
 
 #define GP_GRAPH_STEPS 32
 
 namespace gpk {
+
+// tile shapes of the sparse two-pass path: 0 is the fused kernel's shape (the
+// default), 1.. are the autotuner's further candidates (P <= 4)
+constexpr int kNumShapes = 6;
 
 // per-(P,C,R) entry points, instantiated in gp_inst_p*.hip
 template <int P, int C, class R>

@@ -29,6 +29,29 @@ struct FusedSmem {
   R dp[P][RX1][RY1];
 };
 
+// One grid of a tile set: its parameters, ping-pong fields and scratch planes,
+// its tile-row count and the global index of its first tile.  A context is a
+// set of one grid; an ensemble concatenates the tiles of all its grids, and
+// every tile-list kernel looks its grid up per tile (tinst: global tile ->
+// grid; null when there is one grid).
+template <class R>
+struct Inst {
+  const DevParams<R>* prm;
+  R* phi[2];
+  R* mu[2];
+  R* mulo[2];
+  R* dphi;
+  signed char* code;
+  int nty;
+  int tile0;
+};
+
+template <class R>
+__device__ __forceinline__ const Inst<R>& inst_of(const Inst<R>* __restrict__ inst, const int* __restrict__ tinst,
+                                                  int tile) {
+  return inst[tinst ? tinst[tile] : 0];
+}
+
 // Bulk-tile flag of a cell: the phase a if phi is one-hot in a (phi_a == 1, all
 // others == 0), -1 otherwise; -2 marks "no cell" in reductions.
 template <int P, class R>
@@ -261,17 +284,19 @@ __device__ __forceinline__ void bulk_cell(const DevParams<R>& p, const R* __rest
 // Bulk tiles: the tile and its 2-cell halo are one-hot in one phase, so every
 // cell satisfies the bulk_cell conditions; phi is copied.
 template <int P, int C, class R, int TX, int TY>
-__global__ void __launch_bounds__(256) k_bulk(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
-                                              const R* __restrict__ mu, const R* __restrict__ mulo,
-                                              R* __restrict__ phi_o, R* __restrict__ mu_o, R* __restrict__ mulo_o,
+__global__ void __launch_bounds__(256) k_bulk(const Inst<R>* __restrict__ inst, const int* __restrict__ tinst, int par,
                                               const int* __restrict__ list, const int* __restrict__ count,
-                                              const int* __restrict__ flag_i, int* __restrict__ flag_o, int nty) {
-  const DevParams<R>& p = *prm;
+                                              const int* __restrict__ flag_i, int* __restrict__ flag_o) {
   const int ntile = *count;
   for (int it = blockIdx.x; it < ntile; it += gridDim.x) {
     const int tile = list[it];
-    const int x0 = 3 + (tile / nty) * TX;
-    const int y0 = 3 + (tile % nty) * TY;
+    const Inst<R>& g = inst_of(inst, tinst, tile);
+    const DevParams<R>& p = *g.prm;
+    const int lt = tile - g.tile0;
+    const int x0 = 3 + (lt / g.nty) * TX;
+    const int y0 = 3 + (lt % g.nty) * TY;
+    const R *phi = g.phi[par], *mu = g.mu[par], *mulo = g.mulo[par];
+    R *phi_o = g.phi[par ^ 1], *mu_o = g.mu[par ^ 1], *mulo_o = g.mulo[par ^ 1];
     for (int i = threadIdx.x; i < TX * TY; i += 256) {
       const int x = x0 + i / TY, y = y0 + i % TY;
       if (x > p.mesh_x + 2 || y > p.mesh_y + 2) continue;
@@ -312,19 +337,22 @@ __device__ __forceinline__ bool phase_active(const DevParams<R>& p, const R* __r
 // one-hot in a and delta-phi is +0 in every phase, else -1.  A cell whose 3x3
 // block carries one code a >= 0 meets the bulk_cell conditions.
 template <int P, int C, class R, int TX, int TY>
-__global__ void __launch_bounds__(256) GP_HEAVY k_phase_tiles(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
-                                                     const R* __restrict__ mu, R* __restrict__ dphi,
-                                                     signed char* __restrict__ code,
-                                                     const int* __restrict__ list, const int* __restrict__ count,
-                                                     int nty) {
+__global__ void __launch_bounds__(256) GP_HEAVY k_phase_tiles(const Inst<R>* __restrict__ inst, const int* __restrict__ tinst,
+                                                              int par, const int* __restrict__ list,
+                                                              const int* __restrict__ count) {
   constexpr int RX1 = TX + 2, RY1 = TY + 2;
   __shared__ short todo[RX1 * RY1];
   __shared__ int ntodo;
-  const DevParams<R>& p = *prm;
   const int ntile = *count;
   for (int it = blockIdx.x; it < ntile; it += gridDim.x) {
     const int tile = list[it];
-    const int x0 = 3 + (tile / nty) * TX - 1, y0 = 3 + (tile % nty) * TY - 1;
+    const Inst<R>& g = inst_of(inst, tinst, tile);
+    const DevParams<R>& p = *g.prm;
+    const R *phi = g.phi[par], *mu = g.mu[par];
+    R* dphi = g.dphi;
+    signed char* code = g.code;
+    const int lt = tile - g.tile0;
+    const int x0 = 3 + (lt / g.nty) * TX - 1, y0 = 3 + (lt % g.nty) * TY - 1;
     if (threadIdx.x == 0) ntodo = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < RX1 * RY1; i += 256) {
@@ -364,21 +392,22 @@ __global__ void __launch_bounds__(256) GP_HEAVY k_phase_tiles(const DevParams<R>
 }
 
 template <int P, int C, class R, int TX, int TY>
-__global__ void __launch_bounds__(256) GP_HEAVY k_conc_tiles(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
-                                                    const R* __restrict__ mu, const R* __restrict__ dphi,
-                                                    const signed char* __restrict__ code,
-                                                    const R* __restrict__ mulo, R* __restrict__ phi_o,
-                                                    R* __restrict__ mu_o, R* __restrict__ mulo_o,
-                                                    const int* __restrict__ list, const int* __restrict__ count,
-                                                    int* __restrict__ flag_o, int nty) {
+__global__ void __launch_bounds__(256) GP_HEAVY k_conc_tiles(const Inst<R>* __restrict__ inst, const int* __restrict__ tinst,
+                                                             int par, const int* __restrict__ list,
+                                                             const int* __restrict__ count, int* __restrict__ flag_o) {
   __shared__ int red[256];
   __shared__ short todo[TX * TY];
   __shared__ int ntodo;
-  const DevParams<R>& p = *prm;
   const int ntile = *count;
   for (int it = blockIdx.x; it < ntile; it += gridDim.x) {
     const int tile = list[it];
-    const int x0 = 3 + (tile / nty) * TX, y0 = 3 + (tile % nty) * TY;
+    const Inst<R>& g = inst_of(inst, tinst, tile);
+    const DevParams<R>& p = *g.prm;
+    const R *phi = g.phi[par], *mu = g.mu[par], *mulo = g.mulo[par], *dphi = g.dphi;
+    const signed char* code = g.code;
+    R *phi_o = g.phi[par ^ 1], *mu_o = g.mu[par ^ 1], *mulo_o = g.mulo[par ^ 1];
+    const int lt = tile - g.tile0;
+    const int x0 = 3 + (lt / g.nty) * TX, y0 = 3 + (lt % g.nty) * TY;
     if (threadIdx.x == 0) ntodo = 0;
     __syncthreads();
     int f = -2;
@@ -447,14 +476,17 @@ __global__ void k_plan(const int* __restrict__ flag, const int* __restrict__ dep
   }
 }
 
-// Flags of a freshly loaded state.
+// Flags of a freshly loaded state (one block per tile).
 template <int P, class R, int TX, int TY>
-__global__ void __launch_bounds__(256) k_flags(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
-                                               int* __restrict__ flag, int nty) {
+__global__ void __launch_bounds__(256) k_flags(const Inst<R>* __restrict__ inst, const int* __restrict__ tinst,
+                                               int par, int* __restrict__ flag) {
   __shared__ int red[256];
-  const DevParams<R>& p = *prm;
   const int tile = blockIdx.x;
-  const int x0 = 3 + (tile / nty) * TX, y0 = 3 + (tile % nty) * TY;
+  const Inst<R>& g = inst_of(inst, tinst, tile);
+  const DevParams<R>& p = *g.prm;
+  const R* phi = g.phi[par];
+  const int lt = tile - g.tile0;
+  const int x0 = 3 + (lt / g.nty) * TX, y0 = 3 + (lt % g.nty) * TY;
   int f = -2;
   for (int i = threadIdx.x; i < TX * TY; i += 256) {
     const int x = x0 + i / TY, y = y0 + i % TY;

@@ -20,31 +20,49 @@ struct FusedTile {
   static constexpr int TY = fits<8, 64> || fits<4, 64> ? 64 : 32;
 };
 
-// Host-side tile geometry and ghost-aware dependency lists: tile t depends on
-// every tile owning a source cell of its 2-cell halo, the halo's ghost cells
-// mapped to their copy sources (bc_proxy of phi).
-inline int setup_tiles(gp_ctx *c, int TX, int TY) {
-  if (c->dflag[0] && c->tile_tx == TX && c->tile_ty == TY) return GP_OK;
-  const long MX = c->hp.mesh_x, MY = c->hp.mesh_y;
-  c->tile_tx = TX;
-  c->tile_ty = TY;
-  c->ntx = (int)((MX + TX - 1) / TX);
-  c->nty = (int)((MY + TY - 1) / TY);
-  c->ntiles = c->ntx * c->nty;
-  c->maxdep = 16;
-  const int rows[2] = {c->rx, c->ry};
-  // ghost source along an axis: padded index -> interior padded index
+// Tile shapes of the sparse two-pass path (TX rows x TY columns of cells).
+// Shape 0 is FusedTile; the others are the autotuner's candidates for P <= 4.
+template <int P, int C, class R, int S>
+struct ShapeOf {
+  static constexpr int TX = S == 0 ? FusedTile<P, C, R>::TX : S == 1 ? 4 : S == 2 ? 16 : S == 3 ? 8 : S == 4 ? 32 : 16;
+  static constexpr int TY = S == 0 ? FusedTile<P, C, R>::TY : S == 1 ? 64 : S == 2 ? 64 : S == 3 ? 32 : S == 4 ? 32 : 32;
+};
+template <int P>
+constexpr bool shape_allowed(int s) { return s == 0 || P <= 4; }
+
+inline int shape_tx(int s, int tx0) { return s == 0 ? tx0 : s == 1 ? 4 : s == 2 ? 16 : s == 3 ? 8 : s == 4 ? 32 : 16; }
+inline int shape_ty(int s, int ty0) { return s == 0 ? ty0 : s == 1 ? 64 : s == 2 ? 64 : s == 3 ? 32 : s == 4 ? 32 : 32; }
+
+inline void free_tiles(gp_ctx *c) {
+  for (int i = 0; i < 2; i++)
+    if (c->dflag[i]) (void)hipFree(c->dflag[i]), c->dflag[i] = nullptr;
+  if (c->ddeps) (void)hipFree(c->ddeps), c->ddeps = nullptr;
+  if (c->dlist) (void)hipFree(c->dlist), c->dlist = nullptr;
+  if (c->dlist_all) (void)hipFree(c->dlist_all), c->dlist_all = nullptr;
+  if (c->dcnt) (void)hipFree(c->dcnt), c->dcnt = nullptr;
+  if (c->dinst) (void)hipFree(c->dinst), c->dinst = nullptr;
+  if (c->dtinst) (void)hipFree(c->dtinst), c->dtinst = nullptr;
+  for (int i = 0; i < 2; i++)
+    if (c->graph[i]) (void)hipGraphExecDestroy(c->graph[i]), c->graph[i] = nullptr;
+  c->shape_set = -1;
+}
+
+// Ghost-aware dependency lists of one grid: tile t depends on every tile owning
+// a source cell of its 2-cell halo, the halo's ghost cells mapped to their copy
+// sources (bc_proxy of phi).  Indices are global (offset g->tile0).
+inline void grid_deps(const gp_ctx *g, int TX, int TY, int maxdep, int *deps) {
+  const long MX = g->hp.mesh_x, MY = g->hp.mesh_y;
+  const int rows[2] = {g->rx, g->ry};
   auto src = [&](int axis, int v) -> int {
     long n = axis ? MY : MX;
     if (v >= 3 && v <= n + 2) return v;
-    const long *lo = c->hp.bc_proxy[0][axis ? GP_FACE_YMIN : GP_FACE_XMIN];
-    const long *hi = c->hp.bc_proxy[0][axis ? GP_FACE_YMAX : GP_FACE_XMAX];
-    if (v <= 2) return (int)lo[2 - v];                 // ghosts 2,1,0 <- proxy 0,1,2
-    return (int)hi[v - (rows[axis] - 3)];              // ghosts n-3.. <- proxy 0,1,2
+    const long *lo = g->hp.bc_proxy[0][axis ? GP_FACE_YMIN : GP_FACE_XMIN];
+    const long *hi = g->hp.bc_proxy[0][axis ? GP_FACE_YMAX : GP_FACE_XMAX];
+    if (v <= 2) return (int)lo[2 - v];     // ghosts 2,1,0 <- proxy 0,1,2
+    return (int)hi[v - (rows[axis] - 3)];  // ghosts n-3.. <- proxy 0,1,2
   };
-  std::vector<int> deps((size_t)c->ntiles * (c->maxdep + 1), 0);
-  for (int tx = 0; tx < c->ntx; tx++)
-    for (int ty = 0; ty < c->nty; ty++) {
+  for (int tx = 0; tx < g->ntx; tx++)
+    for (int ty = 0; ty < g->nty; ty++) {
       int sx[64], sy[128], nsx = 0, nsy = 0;
       int x0 = 3 + tx * TX, x1 = (int)std::min<long>(x0 + TX - 1, MX + 2);
       int y0 = 3 + ty * TY, y1 = (int)std::min<long>(y0 + TY - 1, MY + 2);
@@ -52,7 +70,7 @@ inline int setup_tiles(gp_ctx *c, int TX, int TY) {
         int o = (src(0, x) - 3) / TX;
         bool seen = false;
         for (int i = 0; i < nsx; i++) seen |= sx[i] == o;
-        if (!seen) sx[nsx++] = o;
+        if (!seen && nsx < 64) sx[nsx++] = o;
       }
       for (int y = y0 - 2; y <= y1 + 2; y++) {
         int o = (src(1, y) - 3) / TY;
@@ -60,83 +78,146 @@ inline int setup_tiles(gp_ctx *c, int TX, int TY) {
         for (int i = 0; i < nsy; i++) seen |= sy[i] == o;
         if (!seen && nsy < 128) sy[nsy++] = o;
       }
-      int *d = &deps[(size_t)(tx * c->nty + ty) * (c->maxdep + 1)];
-      if (nsx * nsy > c->maxdep) {
+      int *d = &deps[(size_t)(g->tile0 + tx * g->nty + ty) * (maxdep + 1)];
+      if (nsx * nsy > maxdep) {
         d[0] = -1;  // always an interface tile
         continue;
       }
       int n = 0;
       for (int i = 0; i < nsx; i++)
-        for (int j = 0; j < nsy; j++) d[1 + n++] = sx[i] * c->nty + sy[j];
+        for (int j = 0; j < nsy; j++) d[1 + n++] = g->tile0 + sx[i] * g->nty + sy[j];
       d[0] = n;
     }
-  std::vector<int> all(c->ntiles);
-  for (int i = 0; i < c->ntiles; i++) all[i] = i;
-  size_t nt = (size_t)c->ntiles;
+}
+
+// Tile set of a context or ensemble for tile shape TX x TY: per-grid geometry,
+// dependency lists, flags, lists, counters and the device grid table.
+template <class R>
+int setup_tiles(gp_ctx *c, int shape, int TX, int TY) {
+  if (c->dflag[0] && c->shape_set == shape) return GP_OK;
+  free_tiles(c);
+  const int ng = c->nmember ? c->nmember : 1;
+  gp_ctx *const *grids = c->nmember ? c->member : &c;
+  int total = 0;
+  for (int i = 0; i < ng; i++) {
+    gp_ctx *g = grids[i];
+    g->tile_tx = TX;
+    g->tile_ty = TY;
+    g->ntx = (int)((g->hp.mesh_x + TX - 1) / TX);
+    g->nty = (int)((g->hp.mesh_y + TY - 1) / TY);
+    g->ntiles = g->ntx * g->nty;
+    g->tile0 = total;
+    total += g->ntiles;
+  }
+  c->tile_tx = TX;
+  c->tile_ty = TY;
+  c->ntiles = total;
+  c->maxdep = 16;
+  std::vector<int> deps((size_t)total * (c->maxdep + 1), 0);
+  std::vector<int> tinst(c->nmember ? total : 0);
+  std::vector<Inst<R>> inst(ng);
+  for (int i = 0; i < ng; i++) {
+    gp_ctx *g = grids[i];
+    grid_deps(g, TX, TY, c->maxdep, deps.data());
+    for (int t = 0; t < g->ntiles && c->nmember; t++) tinst[g->tile0 + t] = i;
+    Inst<R> &I = inst[i];
+    I.prm = (const DevParams<R> *)g->dprm;
+    for (int k = 0; k < 2; k++) {
+      I.phi[k] = (R *)g->phi[k];
+      I.mu[k] = (R *)g->mu[k];
+      I.mulo[k] = (R *)g->mulo[k];
+    }
+    I.dphi = (R *)g->dphi;
+    I.code = g->dcode;
+    I.nty = g->nty;
+    I.tile0 = g->tile0;
+  }
+  std::vector<int> all(total);
+  for (int i = 0; i < total; i++) all[i] = i;
+  size_t nt = (size_t)total;
   GP_HIP_CHECK(hipMalloc((void **)&c->dflag[0], nt * sizeof(int)));
   GP_HIP_CHECK(hipMalloc((void **)&c->dflag[1], nt * sizeof(int)));
   GP_HIP_CHECK(hipMalloc((void **)&c->ddeps, deps.size() * sizeof(int)));
   GP_HIP_CHECK(hipMalloc((void **)&c->dlist, 2 * nt * sizeof(int)));
   GP_HIP_CHECK(hipMalloc((void **)&c->dlist_all, nt * sizeof(int)));
   GP_HIP_CHECK(hipMalloc((void **)&c->dcnt, 8 * sizeof(int)));
-  int cnt0[8] = {0, 0, 0, 0, c->ntiles, c->ntiles, 0, 0};
+  GP_HIP_CHECK(hipMalloc(&c->dinst, ng * sizeof(Inst<R>)));
+  if (c->nmember) GP_HIP_CHECK(hipMalloc((void **)&c->dtinst, nt * sizeof(int)));
+  int cnt0[8] = {0, 0, 0, 0, total, total, 0, 0};
   GP_HIP_CHECK(hipMemcpy(c->ddeps, deps.data(), deps.size() * sizeof(int), hipMemcpyHostToDevice));
   GP_HIP_CHECK(hipMemcpy(c->dlist_all, all.data(), nt * sizeof(int), hipMemcpyHostToDevice));
   GP_HIP_CHECK(hipMemcpy(c->dcnt, cnt0, sizeof(cnt0), hipMemcpyHostToDevice));
+  GP_HIP_CHECK(hipMemcpy(c->dinst, inst.data(), ng * sizeof(Inst<R>), hipMemcpyHostToDevice));
+  if (c->nmember) GP_HIP_CHECK(hipMemcpy(c->dtinst, tinst.data(), nt * sizeof(int), hipMemcpyHostToDevice));
   c->cnt_par = 0;
   c->flags_valid = 0;
+  c->shape_set = shape;
+  return GP_OK;
+}
+
+// Sparse steps (paths 0, 2, 3) with tile shape S.
+template <int P, int C, class R, int S>
+int run_sparse(gp_ctx *c, long nsteps) {
+  using T = ShapeOf<P, C, R, S>;
+  int rc = setup_tiles<R>(c, S, T::TX, T::TY);
+  if (rc) return rc;
+  const int nt = c->ntiles;
+  const Inst<R> *inst = (const Inst<R> *)c->dinst;
+  const int *tinst = c->dtinst;
+  if (!c->flags_valid) {
+    hipLaunchKernelGGL((k_flags<P, R, T::TX, T::TY>), dim3(nt), dim3(256), 0, c->stream, inst, tinst, c->cur,
+                       c->dflag[c->cur]);
+    c->flags_valid = 1;
+  }
+  const int g_if = std::min(nt, 2048), g_bulk = std::min(nt, 4096);
+  for (long t = 0; t < nsteps; t++) {
+    int i = c->cur, o = c->cur ^ 1;
+    int *cnt = c->dcnt + 2 * c->cnt_par, *cnt_next = c->dcnt + 2 * (c->cnt_par ^ 1);
+    const int *list_if = c->dlist, *cnt_if = cnt;
+    if (c->path != 2) {
+      hipLaunchKernelGGL(k_plan<0>, dim3((nt + 255) / 256), dim3(256), 0, c->stream, (const int *)c->dflag[i],
+                         (const int *)c->ddeps, c->maxdep, nt, c->dlist, c->dlist + nt, cnt, cnt_next);
+    } else {
+      list_if = c->dlist_all;
+      cnt_if = c->dcnt + 4;
+    }
+    if (c->path == 3) {
+      hipLaunchKernelGGL((k_phase_tiles<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, inst, tinst, i,
+                         list_if, cnt_if);
+      hipLaunchKernelGGL((k_conc_tiles<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, inst, tinst, i,
+                         list_if, cnt_if, c->dflag[o]);
+    } else if constexpr (S == 0) {
+      const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
+      hipLaunchKernelGGL((k_fused<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, prm,
+                         (const R *)c->phi[i], (const R *)c->mu[i], (const R *)c->mulo[i], (R *)c->phi[o],
+                         (R *)c->mu[o], (R *)c->mulo[o], list_if, cnt_if, c->dflag[o], c->nty);
+    }
+    if (c->path != 2)
+      hipLaunchKernelGGL((k_bulk<P, C, R, T::TX, T::TY>), dim3(g_bulk), dim3(256), 0, c->stream, inst, tinst, i,
+                         (const int *)(c->dlist + nt), (const int *)(cnt + 1), (const int *)c->dflag[i], c->dflag[o]);
+    c->cur = o;
+    c->cnt_par ^= 1;
+  }
+  GP_HIP_CHECK(hipGetLastError());
   return GP_OK;
 }
 
 template <int P, int C, class R>
 int run_steps(gp_ctx *c, long nsteps) {
-  const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
   if (c->path == 0 || c->path == 2 || c->path == 3) {
-    using T = FusedTile<P, C, R>;
-    int rc = setup_tiles(c, T::TX, T::TY);
-    if (rc) return rc;
-    const int nt = c->ntiles;
-    if (!c->flags_valid) {
-      hipLaunchKernelGGL((k_flags<P, R, T::TX, T::TY>), dim3(nt), dim3(256), 0, c->stream, prm,
-                         (const R *)c->phi[c->cur], c->dflag[c->cur], c->nty);
-      c->flags_valid = 1;
+    // the fused kernel (paths 0, 2) and ensembles of grids use their own constraints
+    int s = (c->path == 3 && shape_allowed<P>(c->shape)) ? c->shape : 0;
+    switch (s) {
+      case 1: if constexpr (shape_allowed<P>(1)) return run_sparse<P, C, R, 1>(c, nsteps); break;
+      case 2: if constexpr (shape_allowed<P>(2)) return run_sparse<P, C, R, 2>(c, nsteps); break;
+      case 3: if constexpr (shape_allowed<P>(3)) return run_sparse<P, C, R, 3>(c, nsteps); break;
+      case 4: if constexpr (shape_allowed<P>(4)) return run_sparse<P, C, R, 4>(c, nsteps); break;
+      case 5: if constexpr (shape_allowed<P>(5)) return run_sparse<P, C, R, 5>(c, nsteps); break;
+      default: break;
     }
-    const int g_if = std::min(nt, 2048), g_bulk = std::min(nt, 4096);
-    for (long t = 0; t < nsteps; t++) {
-      int i = c->cur, o = c->cur ^ 1;
-      int *cnt = c->dcnt + 2 * c->cnt_par, *cnt_next = c->dcnt + 2 * (c->cnt_par ^ 1);
-      const int *list_if = c->dlist, *cnt_if = cnt;
-      if (c->path != 2) {
-        hipLaunchKernelGGL(k_plan<0>, dim3((nt + 255) / 256), dim3(256), 0, c->stream, (const int *)c->dflag[i],
-                           (const int *)c->ddeps, c->maxdep, nt, c->dlist, c->dlist + nt, cnt, cnt_next);
-      } else {
-        list_if = c->dlist_all;
-        cnt_if = c->dcnt + 4;
-      }
-      if (c->path == 3) {
-        hipLaunchKernelGGL((k_phase_tiles<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, prm,
-                           (const R *)c->phi[i], (const R *)c->mu[i], (R *)c->dphi, c->dcode, list_if, cnt_if,
-                           c->nty);
-        hipLaunchKernelGGL((k_conc_tiles<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, prm,
-                           (const R *)c->phi[i], (const R *)c->mu[i], (const R *)c->dphi,
-                           (const signed char *)c->dcode, (const R *)c->mulo[i],
-                           (R *)c->phi[o], (R *)c->mu[o], (R *)c->mulo[o], list_if, cnt_if, c->dflag[o], c->nty);
-      } else {
-        hipLaunchKernelGGL((k_fused<P, C, R, T::TX, T::TY>), dim3(g_if), dim3(256), 0, c->stream, prm,
-                           (const R *)c->phi[i], (const R *)c->mu[i], (const R *)c->mulo[i], (R *)c->phi[o],
-                           (R *)c->mu[o], (R *)c->mulo[o], list_if, cnt_if, c->dflag[o], c->nty);
-      }
-      if (c->path != 2)
-        hipLaunchKernelGGL((k_bulk<P, C, R, T::TX, T::TY>), dim3(g_bulk), dim3(256), 0, c->stream, prm,
-                           (const R *)c->phi[i], (const R *)c->mu[i], (const R *)c->mulo[i], (R *)c->phi[o],
-                           (R *)c->mu[o], (R *)c->mulo[o], (const int *)(c->dlist + nt), (const int *)(cnt + 1),
-                           (const int *)c->dflag[i], c->dflag[o], c->nty);
-      c->cur = o;
-      c->cnt_par ^= 1;
-    }
-    GP_HIP_CHECK(hipGetLastError());
-    return GP_OK;
+    return run_sparse<P, C, R, 0>(c, nsteps);
   }
+  const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
   dim3 blk(BX, BY);
   dim3 g_phase((c->ry - 4 + BX - 1) / BX, (c->rx - 4 + BY - 1) / BY);
   dim3 g_conc((c->hp.mesh_y + BX - 1) / BX, (c->hp.mesh_x + BY - 1) / BY);

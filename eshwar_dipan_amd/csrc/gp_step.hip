@@ -159,6 +159,8 @@ static void free_ctx(gp_ctx *c) {
   if (c->dlist) (void)hipFree(c->dlist);
   if (c->dlist_all) (void)hipFree(c->dlist_all);
   if (c->dcnt) (void)hipFree(c->dcnt);
+  if (c->dinst) (void)hipFree(c->dinst);
+  if (c->dtinst) (void)hipFree(c->dtinst);
   for (int i = 0; i < 2; i++)
     if (c->graph[i]) (void)hipGraphExecDestroy(c->graph[i]);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -167,7 +169,9 @@ static void free_ctx(gp_ctx *c) {
   delete c;
 }
 
-int gp_create(const gp_params *p, int dtype, gp_ctx **out) {
+// A context with its fields; own_stream: create a stream of its own (ensemble
+// members enqueue on their master's stream instead).
+static int ctx_create(const gp_params *p, int dtype, bool own_stream, gp_ctx **out) {
   if (!p || !out || (dtype != GP_F64 && dtype != GP_F32)) return GP_E_ARG;
   *out = nullptr;
   char err[200];
@@ -227,7 +231,7 @@ int gp_create(const gp_params *p, int dtype, gp_ctx **out) {
     return GP_E_NOMEM;
   }
   // a stream of our own (default priority) so that step blocks can be captured into graphs
-  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+  if (own_stream && hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
     free_ctx(c);
     return GP_E_HIP;
   }
@@ -236,9 +240,13 @@ int gp_create(const gp_params *p, int dtype, gp_ctx **out) {
   c->path = 0;
   c->use_graphs = 1;
   c->graph_path = -1;
+  c->shape = 0;
+  c->shape_set = -1;
   *out = c;
   return GP_OK;
 }
+
+int gp_create(const gp_params *p, int dtype, gp_ctx **out) { return ctx_create(p, dtype, true, out); }
 
 void gp_destroy(gp_ctx *c) {
   if (!c) return;
@@ -320,14 +328,15 @@ static int dispatch_steps(gp_ctx *c, long nsteps) {
 // counters) are back where they started and the graph can be replayed.
 static int capture_graph(gp_ctx *c) {
   int slot = c->cur;
-  if (c->graph[slot] && c->graph_path == c->path) return GP_OK;
-  if (c->graph_path != c->path) {
+  const int key = c->path * 16 + c->shape;
+  if (c->graph[slot] && c->graph_path == key) return GP_OK;
+  if (c->graph_path != key) {
     for (int i = 0; i < 2; i++)
       if (c->graph[i]) {
         (void)hipGraphExecDestroy(c->graph[i]);
         c->graph[i] = nullptr;
       }
-    c->graph_path = c->path;
+    c->graph_path = key;
   }
   int cur0 = c->cur, par0 = c->cnt_par;
   hipGraph_t g;
@@ -361,11 +370,71 @@ int gp_step(gp_ctx *c, long nsteps) {
   if (!rc && left > 0) rc = dispatch_steps(c, left);
   if (rc) return rc;
   if (nsteps > 0) c->stepped = 1;
+  for (int i = 0; i < c->nmember; i++) {
+    c->member[i]->cur = c->cur;
+    c->member[i]->stepped = c->stepped;
+  }
   GP_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
   GP_HIP_CHECK(hipEventSynchronize(c->ev1));
   float ms = 0;
   GP_HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
+  return GP_OK;
+}
+
+int gp_set_tile_shape(gp_ctx *c, int shape) {
+  if (!c || shape < 0 || shape >= gpk::kNumShapes) return GP_E_ARG;
+  if (shape != 0 && c->P > 4) return GP_E_UNSUPPORTED;
+  c->shape = shape;
+  return GP_OK;
+}
+
+int gp_tile_shape(gp_ctx *c, int *tile_x, int *tile_y) {
+  if (!c) return GP_E_ARG;
+  int rc = dispatch_steps(c, 0);  // builds the tile set of the current path and shape
+  if (rc) return rc;
+  if (tile_x) *tile_x = c->tile_tx;
+  if (tile_y) *tile_y = c->tile_ty;
+  return GP_OK;
+}
+
+// Time `steps` steps of every candidate (path, tile shape) and keep the
+// fastest.  The state advances by two runs of `steps` steps per candidate; all
+// candidates give bit-identical binary64 results.
+static int autotune(gp_ctx *c, long steps, bool paths) {
+  struct Cand { int path, shape; };
+  Cand cand[gpk::kNumShapes + 2];
+  int n = 0;
+  for (int s = 0; s < gpk::kNumShapes; s++)
+    if (s == 0 || c->P <= 4) cand[n++] = {3, s};
+  if (paths) {
+    cand[n++] = {0, 0};
+    cand[n++] = {1, 0};
+  }
+  double best = 1e300;
+  Cand pick = {c->path, c->shape};
+  for (int i = 0; i < n; i++) {
+    c->path = cand[i].path;
+    c->shape = cand[i].shape;
+    int rc = gp_step(c, steps);  // warm-up: tile set, graph capture
+    if (!rc) rc = gp_step(c, steps);
+    if (rc) return rc;
+    if (c->last_ms < best) {
+      best = c->last_ms;
+      pick = cand[i];
+    }
+  }
+  c->path = pick.path;
+  c->shape = pick.shape;
+  return GP_OK;
+}
+
+int gp_autotune(gp_ctx *c, long steps, int *path, int *shape) {
+  if (!c || steps < 1 || c->nmember) return GP_E_ARG;
+  int rc = autotune(c, steps, true);
+  if (rc) return rc;
+  if (path) *path = c->path;
+  if (shape) *shape = c->shape;
   return GP_OK;
 }
 
@@ -405,6 +474,143 @@ int gp_stats_get(gp_ctx *c, gp_stats *st) {
 #undef GP_DISPATCH_STATS
   if (rc) return rc;
   return st->nonfinite > 0 ? GP_E_NONFINITE : GP_OK;
+}
+
+// ------------------------------------------------------------------ ensembles
+
+struct gp_ensemble {
+  gp_ctx *m;  // master: the tile set over all members, stream, events, graphs
+};
+
+static void free_ensemble(gp_ensemble *e) {
+  if (!e) return;
+  if (e->m) {
+    for (int i = 0; i < e->m->nmember; i++)
+      if (e->m->member[i]) free_ctx(e->m->member[i]);
+    delete[] e->m->member;
+    e->m->nmember = 0;
+    free_ctx(e->m);
+  }
+  delete e;
+}
+
+int gp_ensemble_create(const gp_params *params, int n, int dtype, gp_ensemble **out) {
+  if (!params || !out || n < 1 || (dtype != GP_F64 && dtype != GP_F32)) return GP_E_ARG;
+  *out = nullptr;
+  for (int i = 1; i < n; i++)
+    if (params[i].nphases != params[0].nphases || params[i].ncomp != params[0].ncomp) return GP_E_ARG;
+  if (!pc_supported(params[0].nphases, params[0].ncomp)) return GP_E_UNSUPPORTED;
+  gp_ensemble *e = new (std::nothrow) gp_ensemble();
+  if (!e) return GP_E_NOMEM;
+  gp_ctx *m = e->m = new (std::nothrow) gp_ctx();
+  if (!m) {
+    delete e;
+    return GP_E_NOMEM;
+  }
+  m->member = new (std::nothrow) gp_ctx *[n]();
+  if (!m->member) {
+    free_ensemble(e);
+    return GP_E_NOMEM;
+  }
+  m->nmember = n;
+  m->hp = params[0];
+  m->dtype = dtype;
+  m->P = params[0].nphases;
+  m->C = params[0].ncomp;
+  m->esize = dtype == GP_F64 ? 8 : 4;
+  (void)hipGetDevice(&m->device);
+  if (hipStreamCreateWithFlags(&m->own_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess) {
+    free_ensemble(e);
+    return GP_E_HIP;
+  }
+  m->stream = m->own_stream;
+  m->path = 3;
+  m->use_graphs = 1;
+  m->graph_path = -1;
+  m->shape = 0;
+  m->shape_set = -1;
+  for (int i = 0; i < n; i++) {
+    int rc = ctx_create(&params[i], dtype, false, &m->member[i]);
+    if (rc) {
+      free_ensemble(e);
+      return rc;
+    }
+    m->member[i]->stream = m->stream;
+  }
+  *out = e;
+  return GP_OK;
+}
+
+void gp_ensemble_destroy(gp_ensemble *e) {
+  if (!e) return;
+  if (e->m) (void)hipStreamSynchronize(e->m->stream);
+  free_ensemble(e);
+}
+
+int gp_ensemble_size(gp_ensemble *e, int *n, long *cells) {
+  if (!e) return GP_E_ARG;
+  long s = 0;
+  for (int i = 0; i < e->m->nmember; i++) s += e->m->member[i]->hp.mesh_x * e->m->member[i]->hp.mesh_y;
+  if (n) *n = e->m->nmember;
+  if (cells) *cells = s;
+  return GP_OK;
+}
+
+int gp_ensemble_set_state(gp_ensemble *e, int i, const double *phi, const double *mu) {
+  if (!e || i < 0 || i >= e->m->nmember) return GP_E_ARG;
+  gp_ctx *g = e->m->member[i];
+  g->cur = e->m->cur;
+  int rc = gp_set_state(g, phi, mu);
+  e->m->flags_valid = 0;
+  e->m->stepped = 0;
+  for (int k = 0; k < e->m->nmember; k++) e->m->member[k]->stepped = 0;
+  return rc;
+}
+
+int gp_ensemble_get_state(gp_ensemble *e, int i, double *phi, double *mu) {
+  if (!e || i < 0 || i >= e->m->nmember) return GP_E_ARG;
+  return gp_get_state(e->m->member[i], phi, mu);
+}
+
+int gp_ensemble_step(gp_ensemble *e, long nsteps) {
+  if (!e) return GP_E_ARG;
+  return gp_step(e->m, nsteps);
+}
+
+int gp_ensemble_stats_get(gp_ensemble *e, int i, gp_stats *st) {
+  if (!e || i < 0 || i >= e->m->nmember) return GP_E_ARG;
+  return gp_stats_get(e->m->member[i], st);
+}
+
+int gp_ensemble_last_step_ms(gp_ensemble *e, double *ms) {
+  if (!e || !ms) return GP_E_ARG;
+  *ms = e->m->last_ms;
+  return GP_OK;
+}
+
+int gp_ensemble_set_tile_shape(gp_ensemble *e, int shape) {
+  if (!e) return GP_E_ARG;
+  return gp_set_tile_shape(e->m, shape);
+}
+
+int gp_ensemble_autotune(gp_ensemble *e, long steps, int *tile_x, int *tile_y) {
+  if (!e || steps < 1) return GP_E_ARG;
+  int rc = autotune(e->m, steps, false);
+  if (!rc) rc = gp_tile_shape(e->m, tile_x, tile_y);
+  return rc;
+}
+
+int gp_ensemble_sched_info(gp_ensemble *e, long *ntiles, long *n_interface, long *n_bulk, int *tile_x, int *tile_y) {
+  if (!e) return GP_E_ARG;
+  int rc = gp_tile_shape(e->m, tile_x, tile_y);
+  if (!rc) rc = gp_sched_info(e->m, ntiles, n_interface, n_bulk);
+  return rc;
+}
+
+int gp_ensemble_synchronize(gp_ensemble *e) {
+  if (!e) return GP_E_ARG;
+  return gp_synchronize(e->m);
 }
 
 // sizes of the ABI structures, so bindings can check their layouts

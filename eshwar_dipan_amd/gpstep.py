@@ -55,6 +55,24 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = i
+        pi = ct.POINTER(ct.c_int)
+        pl = ct.POINTER(l)
+        for name, args in [
+            ("gp_set_tile_shape", [vp, i]), ("gp_tile_shape", [vp, pi, pi]),
+            ("gp_autotune", [vp, l, pi, pi]),
+            ("gp_ensemble_create", [ct.POINTER(GpParams), i, i, ct.POINTER(vp)]),
+            ("gp_ensemble_size", [vp, pi, pl]),
+            ("gp_ensemble_set_state", [vp, i, vp, vp]), ("gp_ensemble_get_state", [vp, i, vp, vp]),
+            ("gp_ensemble_step", [vp, l]), ("gp_ensemble_stats_get", [vp, i, ct.POINTER(GpStats)]),
+            ("gp_ensemble_last_step_ms", [vp, ct.POINTER(d)]), ("gp_ensemble_set_tile_shape", [vp, i]),
+            ("gp_ensemble_autotune", [vp, l, pi, pi]),
+            ("gp_ensemble_sched_info", [vp, pl, pl, pl, pi, pi]), ("gp_ensemble_synchronize", [vp]),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = i
+        L.gp_ensemble_destroy.argtypes = [vp]
+        L.gp_ensemble_destroy.restype = None
         L.gp_destroy.argtypes = [vp]
         L.gp_destroy.restype = None
         L.gp_mu_of_c.argtypes = [ct.POINTER(GpParams), vp, d, i, i]
@@ -191,16 +209,119 @@ class Solver:
         _check(lib().gp_last_step_ms(self._h, ct.byref(v)))
         return v.value
 
+    def set_tile_shape(self, shape):
+        _check(lib().gp_set_tile_shape(self._h, int(shape)), "gp_set_tile_shape")
+
+    def tile_shape(self):
+        tx, ty = ct.c_int(), ct.c_int()
+        _check(lib().gp_tile_shape(self._h, ct.byref(tx), ct.byref(ty)), "gp_tile_shape")
+        return tx.value, ty.value
+
+    def autotune(self, steps=64):
+        """Pick the fastest kernel path and tile shape (advances the state by
+        2*steps per candidate); returns (path, shape)."""
+        pa, sh = ct.c_int(), ct.c_int()
+        _check(lib().gp_autotune(self._h, int(steps), ct.byref(pa), ct.byref(sh)), "gp_autotune")
+        return pa.value, sh.value
+
     def stats(self, allow_nonfinite=False):
         st = GpStats()
         rc = lib().gp_stats_get(self._h, ct.byref(st))
         if rc != 0 and not (allow_nonfinite and rc == -5):
             raise GPError(rc, "gp_stats_get")
-        P, C = self.params.nphases, self.params.ncomp
-        return {
-            "phi_max": list(st.phi_max[:P]), "phi_min": list(st.phi_min[:P]),
-            "phi_sum": list(st.phi_sum[:P]), "mu_max": list(st.mu_max[:C]),
-            "mu_min": list(st.mu_min[:C]), "solute": list(st.solute[:C]),
-            "nonfinite": st.nonfinite,
-            "rel_change_phi": list(st.rel_change_phi[:P]), "rel_change_mu": list(st.rel_change_mu[:C]),
-        }
+        return _stats_dict(st, self.params)
+
+
+def _stats_dict(st, params):
+    P, C = params.nphases, params.ncomp
+    return {
+        "phi_max": list(st.phi_max[:P]), "phi_min": list(st.phi_min[:P]),
+        "phi_sum": list(st.phi_sum[:P]), "mu_max": list(st.mu_max[:C]),
+        "mu_min": list(st.mu_min[:C]), "solute": list(st.solute[:C]),
+        "nonfinite": st.nonfinite,
+        "rel_change_phi": list(st.rel_change_phi[:P]), "rel_change_mu": list(st.rel_change_mu[:C]),
+    }
+
+
+class Ensemble:
+    """Independent grids with the same phase and component counts stepped as one
+    tile set (gp_ensemble_*).  Member i is addressed by index."""
+
+    def __init__(self, params_list, dtype="f64"):
+        ps = [q.p if isinstance(q, GpInput) else q for q in params_list]
+        self.params = ps
+        self.dtype = dtype
+        arr = (GpParams * len(ps))(*ps)
+        h = ct.c_void_p()
+        _check(lib().gp_ensemble_create(arr, len(ps), GP_F64 if dtype == "f64" else GP_F32, ct.byref(h)),
+               "gp_ensemble_create")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().gp_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.params)
+
+    def cells(self):
+        n, c = ct.c_int(), ct.c_long()
+        _check(lib().gp_ensemble_size(self._h, ct.byref(n), ct.byref(c)), "gp_ensemble_size")
+        return c.value
+
+    def set_state(self, i, phi, mu):
+        p = self.params[i]
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        if phi.shape != (p.nphases, p.mesh_x, p.mesh_y) or mu.shape != (p.ncomp, p.mesh_x, p.mesh_y):
+            raise ValueError(f"member {i}: state shapes {phi.shape}, {mu.shape}")
+        _check(lib().gp_ensemble_set_state(self._h, int(i), _ptr(phi), _ptr(mu)), "gp_ensemble_set_state")
+
+    def get_state(self, i):
+        p = self.params[i]
+        phi = np.empty((p.nphases, p.mesh_x, p.mesh_y))
+        mu = np.empty((p.ncomp, p.mesh_x, p.mesh_y))
+        _check(lib().gp_ensemble_get_state(self._h, int(i), _ptr(phi), _ptr(mu)), "gp_ensemble_get_state")
+        return phi, mu
+
+    def step(self, n=1):
+        _check(lib().gp_ensemble_step(self._h, int(n)), "gp_ensemble_step")
+
+    def stats(self, i, allow_nonfinite=False):
+        st = GpStats()
+        rc = lib().gp_ensemble_stats_get(self._h, int(i), ct.byref(st))
+        if rc != 0 and not (allow_nonfinite and rc == -5):
+            raise GPError(rc, "gp_ensemble_stats_get")
+        return _stats_dict(st, self.params[i])
+
+    def last_step_ms(self):
+        v = ct.c_double()
+        _check(lib().gp_ensemble_last_step_ms(self._h, ct.byref(v)))
+        return v.value
+
+    def set_tile_shape(self, shape):
+        _check(lib().gp_ensemble_set_tile_shape(self._h, int(shape)), "gp_ensemble_set_tile_shape")
+
+    def autotune(self, steps=32):
+        """Pick the fastest tile shape (advances the state); returns (tile_x, tile_y)."""
+        tx, ty = ct.c_int(), ct.c_int()
+        _check(lib().gp_ensemble_autotune(self._h, int(steps), ct.byref(tx), ct.byref(ty)), "gp_ensemble_autotune")
+        return tx.value, ty.value
+
+    def sched_info(self):
+        """(tiles, interface tiles, bulk tiles, tile_x, tile_y) of the last step."""
+        a, b, c = ct.c_long(), ct.c_long(), ct.c_long()
+        tx, ty = ct.c_int(), ct.c_int()
+        _check(lib().gp_ensemble_sched_info(self._h, ct.byref(a), ct.byref(b), ct.byref(c), ct.byref(tx),
+                                            ct.byref(ty)), "gp_ensemble_sched_info")
+        return a.value, b.value, c.value, tx.value, ty.value
+
+    def synchronize(self):
+        _check(lib().gp_ensemble_synchronize(self._h), "gp_ensemble_synchronize")

@@ -29,7 +29,7 @@ from . import gpstep, synth
 
 
 def params(P, C, mesh, aniso=0.0, W=1, bc=(1, 1, 1, 1), solver=None, gabc=None, rot=None,
-           dt=None, eps=None, D_sol=0.0, A_off=0.0, T=None, gamma=None, extra=()):
+           dt=None, eps=None, D_sol=0.0, A_off=0.0, T=None, gamma=None, tau=None, extra=()):
     """GpInput for a synthetic configuration, written as Input.in text and read
     back through gp_read_input + gp_derive."""
     keys = synth.base_params(P, C, mesh)
@@ -48,6 +48,9 @@ def params(P, C, mesh, aniso=0.0, W=1, bc=(1, 1, 1, 1), solver=None, gabc=None, 
         keys["T"] = str(T)
     if gamma is not None:
         keys["GAMMA"] = "{" + ",".join([str(gamma)] * npair) + "}"
+    if tau is not None:
+        keys["Tau"] = "{" + ",".join([repr(float(tau))] * npair) + "}"
+        keys["tau"] = repr(float(tau))
     solver = solver or ("BINARY" if C == 1 else "TERNARY" if C == 2 else "GENERAL")
     lines = synth.thermo_lines(P, C, solver=solver, D_sol=D_sol, A_off=A_off)
     lines += synth.bc_lines(bc, bc)
@@ -120,6 +123,31 @@ def c5(mesh=512):
     phi[0] = 0.5 * (1 - np.tanh((r - 8.0) / 1.5))
     phi[1] = 1.0 - phi[0]
     return inp, phi, mu, 2000, f"2-phase binary dendrite {mesh}x{mesh}, 4-fold anisotropy 0.02"
+
+
+def c4(n=512, steps=2000, seed=4):
+    """Ensemble of n 2-phase binary grids: side drawn from {64, 96, ..., 512},
+    undercooling T_eq - T in [0.05, 0.3], interface mobility M in [0.5, 2]
+    (Tau = 1/M for the pair and the scalar tau), anisotropy strength in
+    [0, 0.05]; one seed of radius 10 at the centre of each grid, Neumann
+    boundaries.  Returns ([(GpInput, phi, mu)], steps, description)."""
+    sides = list(range(64, 513, 32))
+    u = synth.splitmix_uniforms(seed, 4 * n).reshape(n, 4)
+    members = []
+    for i in range(n):
+        side = sides[min(int(u[i, 0] * len(sides)), len(sides) - 1)]
+        dT = 0.05 + 0.25 * u[i, 1]
+        M = 0.5 + 1.5 * u[i, 2]
+        delta = 0.05 * u[i, 3]
+        inp = params(2, 1, (side, side), aniso=max(delta, 1e-12), T=1.0 - dT, gamma=0.5, tau=1.0 / M)
+        phi, mu = seeded_state(inp, 0, 1, 1, seed=seed)
+        X, Y = np.meshgrid(np.arange(side), np.arange(side), indexing="ij")
+        disk = (X - side // 2) ** 2 + (Y - side // 2) ** 2 <= 10 ** 2
+        phi[0][disk], phi[1][disk] = 1.0, 0.0
+        members.append((inp, phi, mu))
+    desc = (f"ensemble of {n} 2-phase binary grids, sides 64..512 step 32, undercooling 0.05-0.3, "
+            "mobility 0.5-2, anisotropy 0-0.05, one seed r=10 each")
+    return members, steps, desc
 
 
 WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c5": c5}

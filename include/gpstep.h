@@ -161,7 +161,9 @@ int  gp_last_step_ms(gp_ctx *ctx, double *ms);
 /* Kernel path of a context: 0 = fused LDS kernel on interface tiles plus a
    diffusion-only kernel on bulk tiles (default), 1 = two-pass kernels (phase
    kernel + concentration kernel over the whole grid), 2 = fused LDS kernel on
-   every tile.  All paths give bit-identical binary64 results. */
+   every tile, 3 = two-pass kernels on interface tiles (cells sorted in LDS into
+   interface and bulk cells) plus the bulk kernel.  All paths give bit-identical
+   binary64 results. */
 int  gp_set_path(gp_ctx *ctx, int path);
 
 /* Replay gp_step calls of 32 steps or more as captured HIP graphs of 32-step
@@ -172,6 +174,39 @@ int  gp_set_graphs(gp_ctx *ctx, int on);
 /* Tiles of the last fused step: total, interface (full kernel) and bulk
    (diffusion-only kernel) counts; synchronises the context's stream. */
 int  gp_sched_info(gp_ctx *ctx, long *ntiles, long *n_interface, long *n_bulk);
+
+/* Tile shape of the sparse two-pass path: 0 is the default; 1..5 are further
+   candidates (P <= 4): (4,64), (16,64), (8,32), (32,32), (16,32) cells. */
+int  gp_set_tile_shape(gp_ctx *ctx, int shape);
+/* Rows x columns of cells of the tile shape the context steps with. */
+int  gp_tile_shape(gp_ctx *ctx, int *tile_x, int *tile_y);
+
+/* Time `steps` steps of every candidate kernel path and tile shape and keep the
+   fastest.  The state advances by 2 x steps per candidate; binary64 results do
+   not depend on the choice. */
+int  gp_autotune(gp_ctx *ctx, long steps, int *path, int *shape);
+
+/*
+ * Ensembles: n independent grids with the same P and C (sizes, constants and
+ * boundaries may differ) stepped together as one tile set, so that each step of
+ * the whole ensemble is one planner launch and one launch per kernel.  Blocks of
+ * 32 steps are replayed as one captured HIP graph.  Member i is addressed by
+ * its index; its state and statistics work as for a context.  Each member's
+ * binary64 results equal those of a context of its own.
+ */
+typedef struct gp_ensemble gp_ensemble;
+int  gp_ensemble_create(const gp_params *params, int n, int dtype, gp_ensemble **out);
+void gp_ensemble_destroy(gp_ensemble *e);
+int  gp_ensemble_size(gp_ensemble *e, int *n, long *cells);
+int  gp_ensemble_set_state(gp_ensemble *e, int i, const double *phi, const double *mu);
+int  gp_ensemble_get_state(gp_ensemble *e, int i, double *phi, double *mu);
+int  gp_ensemble_step(gp_ensemble *e, long nsteps);
+int  gp_ensemble_stats_get(gp_ensemble *e, int i, gp_stats *st);
+int  gp_ensemble_last_step_ms(gp_ensemble *e, double *ms);
+int  gp_ensemble_set_tile_shape(gp_ensemble *e, int shape);
+int  gp_ensemble_autotune(gp_ensemble *e, long steps, int *tile_x, int *tile_y);
+int  gp_ensemble_sched_info(gp_ensemble *e, long *ntiles, long *n_interface, long *n_bulk, int *tile_x, int *tile_y);
+int  gp_ensemble_synchronize(gp_ensemble *e);
 
 const char *gp_strerror(int code);
 
