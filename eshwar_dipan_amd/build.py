@@ -40,7 +40,7 @@ def _units():
     u = []
     u.append((os.path.join(BUILD, "gp_host.o"),
               ["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "gp_host.c")]))
-    for name in ("gp_step",):
+    for name in ("gp_step", "gp_adjoint"):
         u.append((os.path.join(BUILD, name + ".o"),
                   [HIPCC, *HIP_FLAGS, "-ffp-contract=off", "-c", os.path.join(CSRC, name + ".hip")]))
     for P in INST_P:

@@ -1,0 +1,481 @@
+// gp_adjoint.hip -- gradients of a field-mismatch loss through n explicit steps
+// (include/gpstep.h, gp_adjoint_*): binomial checkpointing over time on the host,
+// forward steps through the context's own kernels, reverse steps through the
+// gather kernels below (cell programs in gp_adjoint.hpp).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "gp_adjoint.hpp"
+#include "gp_ctx.hpp"
+
+namespace gpk {
+
+// planes of the per-direction partials
+constexpr int CG_PHI = 0;   // 9 directions x 2 phases
+constexpr int CG_MU = 18;   // 5 directions
+constexpr int CG_DP = 23;   // 5 directions x 2 phases
+constexpr int CG_N = 33;
+constexpr int CF_PHI = 0;   // 9 x 2
+constexpr int CF_MU = 18;   // own cell
+constexpr int CF_N = 19;
+
+constexpr int ABX = 64, ABY = 4;
+
+__device__ __forceinline__ void block_sum4(double* v, double* red) {
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int k = 0; k < ADJ_NPAR; k++) red[k * 256 + t] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+      for (int k = 0; k < ADJ_NPAR; k++) red[k * 256 + t] += red[k * 256 + t + s];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int k = 0; k < ADJ_NPAR; k++) v[k] = red[k * 256];
+}
+
+template <class R>
+__device__ __forceinline__ void load_v(const DevParams<R>& p, const R* phi, int x, int y, R (&v)[2][3][3]) {
+  for (int a = 0; a < 2; a++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) v[a][i][j] = phi[idx(p, a, x + i - 1, y + j - 1)];
+}
+
+// reverse of the mu/phi update, one thread per interior cell
+template <class R>
+__global__ void __launch_bounds__(256) k_adj_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                                  const R* __restrict__ mu, const R* __restrict__ dphi,
+                                                  const R* __restrict__ lam_phi, const R* __restrict__ lam_mu,
+                                                  R* __restrict__ cg, double* __restrict__ part) {
+  __shared__ double red[ADJ_NPAR * 256];
+  const DevParams<R>& p = *prm;
+  const int y = 3 + blockIdx.x * ABX + threadIdx.x;
+  const int x = 3 + blockIdx.y * ABY + threadIdx.y;
+  double par[ADJ_NPAR] = {0, 0, 0, 0};
+  if (x <= p.mesh_x + 2 && y <= p.mesh_y + 2) {
+    const size_t o = (size_t)x * p.pitch + y;
+    R seedp[2] = {lam_phi[o], lam_phi[p.plane + o]};
+    R seedm = lam_mu[o];
+    R vb[2][3][3] = {}, mub5[5] = {}, dpb5[5][2] = {};
+    if (seedp[0] != R(0) || seedp[1] != R(0) || seedm != R(0)) {
+      R v[2][3][3], mu5[5], dp5[5][2];
+      load_v(p, phi, x, y, v);
+      for (int d = 0; d < 5; d++) {
+        mu5[d] = mu[idx(p, 0, x + D5X[d], y + D5Y[d])];
+        for (int a = 0; a < 2; a++) dp5[d][a] = dphi[idx(p, a, x + D5X[d], y + D5Y[d])];
+      }
+      adj_conc_cell(p, v, mu5, dp5, seedp, seedm, y, vb, mub5, dpb5, par);
+    }
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        for (int a = 0; a < 2; a++) cg[(size_t)(CG_PHI + (i * 3 + j) * 2 + a) * p.plane + o] = vb[a][i][j];
+    for (int d = 0; d < 5; d++) {
+      cg[(size_t)(CG_MU + d) * p.plane + o] = mub5[d];
+      for (int a = 0; a < 2; a++) cg[(size_t)(CG_DP + d * 2 + a) * p.plane + o] = dpb5[d][a];
+    }
+  }
+  block_sum4(par, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    for (int k = 0; k < ADJ_NPAR; k++) part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * ADJ_NPAR + k] = par[k];
+}
+
+__device__ __forceinline__ bool in_interior(int x, int y, int mx, int my) { return x >= 3 && x <= mx + 2 && y >= 3 && y <= my + 2; }
+
+// adjoint of delta-phi gathered from the update stencils, then the reverse of
+// the phase sweep, one thread per cell of [2, rows-3]^2
+template <class R>
+__global__ void __launch_bounds__(256) k_adj_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                                   const R* __restrict__ mu, const R* __restrict__ cg,
+                                                   R* __restrict__ cf, double* __restrict__ part) {
+  __shared__ double red[ADJ_NPAR * 256];
+  const DevParams<R>& p = *prm;
+  const int y = 2 + blockIdx.x * ABX + threadIdx.x;
+  const int x = 2 + blockIdx.y * ABY + threadIdx.y;
+  double par[ADJ_NPAR] = {0, 0, 0, 0};
+  if (x <= p.rx - 3 && y <= p.ry - 3) {
+    const size_t o = (size_t)x * p.pitch + y;
+    R seed[2] = {R(0), R(0)};
+    for (int d = 0; d < 5; d++) {
+      const int cx = x - D5X[d], cy = y - D5Y[d];
+      if (!in_interior(cx, cy, p.mesh_x, p.mesh_y)) continue;
+      const size_t oc = (size_t)cx * p.pitch + cy;
+      for (int a = 0; a < 2; a++) seed[a] += cg[(size_t)(CG_DP + d * 2 + a) * p.plane + oc];
+    }
+    R vb[2][3][3] = {}, mub = R(0);
+    if (seed[0] != R(0) || seed[1] != R(0)) {
+      R v[2][3][3];
+      load_v(p, phi, x, y, v);
+      adj_phase_cell(p, v, mu[o], seed, vb, mub, par);
+    }
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        for (int a = 0; a < 2; a++) cf[(size_t)(CF_PHI + (i * 3 + j) * 2 + a) * p.plane + o] = vb[a][i][j];
+    cf[(size_t)CF_MU * p.plane + o] = mub;
+  }
+  block_sum4(par, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    for (int k = 0; k < ADJ_NPAR; k++) part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * ADJ_NPAR + k] = par[k];
+}
+
+// sum of the stencil partials pointing at cell j, for j in [1, rows-2]^2
+template <class R>
+__global__ void __launch_bounds__(256) k_adj_gather(const DevParams<R>* __restrict__ prm, const R* __restrict__ cg,
+                                                    const R* __restrict__ cf, R* __restrict__ lt) {
+  const DevParams<R>& p = *prm;
+  const int y = 1 + blockIdx.x * ABX + threadIdx.x;
+  const int x = 1 + blockIdx.y * ABY + threadIdx.y;
+  if (x > p.rx - 2 || y > p.ry - 2) return;
+  R sp[2] = {R(0), R(0)}, sm = R(0);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      const int cx = x - (i - 1), cy = y - (j - 1);
+      const size_t oc = (size_t)cx * p.pitch + cy;
+      const int dir = i * 3 + j;
+      if (in_interior(cx, cy, p.mesh_x, p.mesh_y))
+        for (int a = 0; a < 2; a++) sp[a] += cg[(size_t)(CG_PHI + dir * 2 + a) * p.plane + oc];
+      if (cx >= 2 && cx <= p.rx - 3 && cy >= 2 && cy <= p.ry - 3)
+        for (int a = 0; a < 2; a++) sp[a] += cf[(size_t)(CF_PHI + dir * 2 + a) * p.plane + oc];
+    }
+  for (int d = 0; d < 5; d++) {
+    const int cx = x - D5X[d], cy = y - D5Y[d];
+    if (in_interior(cx, cy, p.mesh_x, p.mesh_y)) sm += cg[(size_t)(CG_MU + d) * p.plane + (size_t)cx * p.pitch + cy];
+  }
+  const size_t o = (size_t)x * p.pitch + y;
+  if (x >= 2 && x <= p.rx - 3 && y >= 2 && y <= p.ry - 3) sm += cf[(size_t)CF_MU * p.plane + o];
+  lt[o] = sp[0];
+  lt[p.plane + o] = sp[1];
+  lt[2 * p.plane + o] = sm;
+}
+
+// reverse of the ghost copies: an interior cell collects the adjoints of its
+// images (the cells write_with_ghosts writes from it)
+template <class R>
+__global__ void __launch_bounds__(256) k_adj_fold(const DevParams<R>* __restrict__ prm, const R* __restrict__ lt,
+                                                  R* __restrict__ lam_phi, R* __restrict__ lam_mu) {
+  const DevParams<R>& p = *prm;
+  const int y = 3 + blockIdx.x * ABX + threadIdx.x;
+  const int x = 3 + blockIdx.y * ABY + threadIdx.y;
+  if (x > p.mesh_x + 2 || y > p.mesh_y + 2) return;
+  for (int f = 0; f < 2; f++) {
+    int tx[3], ty[3], nx = 1, ny = 1;
+    tx[0] = x;
+    ty[0] = y;
+    for (int s = 0; s < 6; s++) {
+      if (p.gsrc[f][0][s] == x) tx[nx++] = ghost_index(s, p.rx);
+      if (p.gsrc[f][1][s] == y) ty[ny++] = ghost_index(s, p.ry);
+    }
+    const int nfield = f == 0 ? 2 : 1;
+    for (int k = 0; k < nfield; k++) {
+      const int plane = f == 0 ? k : 2;
+      R s = R(0);
+      for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+          const int gx = tx[i], gy = ty[j];
+          if (gx < 1 || gx > p.rx - 2 || gy < 1 || gy > p.ry - 2) continue;  // never read by a stencil
+          s += lt[(size_t)plane * p.plane + (size_t)gx * p.pitch + gy];
+        }
+      if (f == 0) lam_phi[(size_t)k * p.plane + (size_t)x * p.pitch + y] = s;
+      else lam_mu[(size_t)x * p.pitch + y] = s;
+    }
+  }
+}
+
+// fixed-order sum of the block partials into the gradient accumulator
+__global__ void k_adj_reduce(const double* __restrict__ part, int nblk, double* __restrict__ acc) {
+  __shared__ double red[ADJ_NPAR * 256];
+  const int t = threadIdx.x;
+  double v[ADJ_NPAR] = {0, 0, 0, 0};
+  for (int b = t; b < nblk; b += 256)
+    for (int k = 0; k < ADJ_NPAR; k++) v[k] += part[(size_t)b * ADJ_NPAR + k];
+  for (int k = 0; k < ADJ_NPAR; k++) red[k * 256 + t] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+      for (int k = 0; k < ADJ_NPAR; k++) red[k * 256 + t] += red[k * 256 + t + s];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int k = 0; k < ADJ_NPAR; k++) acc[k] += red[k * 256];
+}
+
+}  // namespace gpk
+
+struct gp_adjoint {
+  gp_ctx *c;
+  long nsteps;
+  int nslots;
+  size_t slot_bytes;     // bytes of one saved state (phi, mu and mu low words, padded planes)
+  std::vector<void *> slots;
+  void *cg, *cf, *lt;    // partial planes, gathered adjoints
+  void *lam[2][2];       // [buffer][0 = phi (2 planes), 1 = mu]
+  int lcur;
+  double *part, *acc;
+  int nblk;
+  hipEvent_t e0, e1;
+  double fwd_ms, rev_ms;
+  long fwd_steps;
+  // run state
+  bool seeded;
+  const double *target;
+  double loss;
+};
+
+static int adj_save(gp_adjoint *A, int slot) {
+  gp_ctx *c = A->c;
+  const size_t pb = (size_t)c->plane * c->esize;
+  char *d = (char *)A->slots[slot];
+  GP_HIP_CHECK(hipMemcpyAsync(d, c->phi[c->cur], pb * c->P, hipMemcpyDeviceToDevice, c->stream));
+  GP_HIP_CHECK(hipMemcpyAsync(d + pb * c->P, c->mu[c->cur], pb * c->C, hipMemcpyDeviceToDevice, c->stream));
+  if (c->mulo[c->cur])
+    GP_HIP_CHECK(hipMemcpyAsync(d + pb * (c->P + c->C), c->mulo[c->cur], pb * c->C, hipMemcpyDeviceToDevice, c->stream));
+  return GP_OK;
+}
+
+static int adj_restore(gp_adjoint *A, int slot) {
+  gp_ctx *c = A->c;
+  const size_t pb = (size_t)c->plane * c->esize;
+  const char *d = (const char *)A->slots[slot];
+  GP_HIP_CHECK(hipMemcpyAsync(c->phi[c->cur], d, pb * c->P, hipMemcpyDeviceToDevice, c->stream));
+  GP_HIP_CHECK(hipMemcpyAsync(c->mu[c->cur], d + pb * c->P, pb * c->C, hipMemcpyDeviceToDevice, c->stream));
+  if (c->mulo[c->cur])
+    GP_HIP_CHECK(hipMemcpyAsync(c->mulo[c->cur], d + pb * (c->P + c->C), pb * c->C, hipMemcpyDeviceToDevice, c->stream));
+  c->flags_valid = 0;
+  c->stepped = 0;
+  return GP_OK;
+}
+
+static int adj_advance(gp_adjoint *A, long n) {
+  if (n <= 0) return GP_OK;
+  int rc = gp_step(A->c, n);
+  A->fwd_ms += A->c->last_ms;
+  A->fwd_steps += n;
+  return rc;
+}
+
+// the loss and its seed from the state after the last step
+template <class R>
+static int adj_seed(gp_adjoint *A) {
+  gp_ctx *c = A->c;
+  const long MX = c->hp.mesh_x, MY = c->hp.mesh_y;
+  std::vector<double> phi((size_t)2 * MX * MY), mu((size_t)MX * MY);
+  int rc = gp_get_state(c, phi.data(), mu.data());
+  if (rc) return rc;
+  std::vector<R> seed((size_t)2 * c->plane, R(0));
+  double L = 0;
+  for (long x = 0; x < MX; x++)
+    for (long y = 0; y < MY; y++) {
+      const double d = phi[(size_t)x * MY + y] - A->target[(size_t)x * MY + y];
+      L += 0.5 * d * d;
+      seed[(size_t)(x + 3) * c->pitch + (y + 3)] = (R)d;
+    }
+  A->loss = L;
+  std::vector<R> zero((size_t)c->plane, R(0));
+  A->lcur = 0;
+  GP_HIP_CHECK(hipMemcpy(A->lam[0][0], seed.data(), seed.size() * sizeof(R), hipMemcpyHostToDevice));
+  GP_HIP_CHECK(hipMemcpy(A->lam[0][1], zero.data(), zero.size() * sizeof(R), hipMemcpyHostToDevice));
+  A->seeded = true;
+  return GP_OK;
+}
+
+// one reverse step from the context's current state S_k: lam <- J_k^T lam
+template <class R>
+static int adj_step(gp_adjoint *A) {
+  gp_ctx *c = A->c;
+  using namespace gpk;
+  const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
+  const R *phi = (const R *)c->phi[c->cur], *mu = (const R *)c->mu[c->cur];
+  GP_HIP_CHECK(hipEventRecord(A->e0, c->stream));
+  dim3 blk(BX, BY);
+  dim3 g_phase((c->ry - 4 + BX - 1) / BX, (c->rx - 4 + BY - 1) / BY);
+  hipLaunchKernelGGL((k_phase<2, 1, R>), g_phase, blk, 0, c->stream, prm, phi, mu, (R *)c->dphi);
+  const int MX = (int)c->hp.mesh_x, MY = (int)c->hp.mesh_y;
+  dim3 ab(ABX, ABY);
+  dim3 g_in((MY + ABX - 1) / ABX, (MX + ABY - 1) / ABY);
+  dim3 g_ring((c->ry - 4 + ABX - 1) / ABX, (c->rx - 4 + ABY - 1) / ABY);
+  dim3 g_all((c->ry - 2 + ABX - 1) / ABX, (c->rx - 2 + ABY - 1) / ABY);
+  const int li = A->lcur, lo = A->lcur ^ 1;
+  const int nb_in = (int)(g_in.x * g_in.y), nb_ring = (int)(g_ring.x * g_ring.y);
+  hipLaunchKernelGGL((k_adj_conc<R>), g_in, ab, 0, c->stream, prm, phi, mu, (const R *)c->dphi,
+                     (const R *)A->lam[li][0], (const R *)A->lam[li][1], (R *)A->cg, A->part);
+  hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(256), 0, c->stream, (const double *)A->part, nb_in, A->acc);
+  hipLaunchKernelGGL((k_adj_phase<R>), g_ring, ab, 0, c->stream, prm, phi, mu, (const R *)A->cg, (R *)A->cf,
+                     A->part);
+  hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(256), 0, c->stream, (const double *)A->part, nb_ring, A->acc);
+  hipLaunchKernelGGL((k_adj_gather<R>), g_all, ab, 0, c->stream, prm, (const R *)A->cg, (const R *)A->cf,
+                     (R *)A->lt);
+  hipLaunchKernelGGL((k_adj_fold<R>), g_in, ab, 0, c->stream, prm, (const R *)A->lt, (R *)A->lam[lo][0],
+                     (R *)A->lam[lo][1]);
+  GP_HIP_CHECK(hipGetLastError());
+  GP_HIP_CHECK(hipEventRecord(A->e1, c->stream));
+  GP_HIP_CHECK(hipEventSynchronize(A->e1));
+  float ms = 0;
+  GP_HIP_CHECK(hipEventElapsedTime(&ms, A->e0, A->e1));
+  A->rev_ms += ms;
+  A->lcur = lo;
+  return GP_OK;
+}
+
+static int adj_step_any(gp_adjoint *A) {
+  return A->c->dtype == GP_F64 ? adj_step<double>(A) : adj_step<float>(A);
+}
+
+static long binom(int s, int t) {
+  // C(s + t, s), saturating
+  double r = 1;
+  for (int i = 1; i <= s; i++) {
+    r = r * (t + i) / i;
+    if (r > 1e15) return (long)1e15;
+  }
+  return (long)(r + 0.5);
+}
+
+// Reverse step k with the state S_lo in slot `sl`: recompute S_k, seed the
+// adjoint from S_{k+1} first if this is the last step, then reverse.
+static int adj_one(gp_adjoint *A, long lo, long k, int sl) {
+  int rc;
+  if ((rc = adj_restore(A, sl))) return rc;
+  if ((rc = adj_advance(A, k - lo))) return rc;
+  if (!A->seeded) {
+    if ((rc = adj_advance(A, 1))) return rc;
+    rc = A->c->dtype == GP_F64 ? adj_seed<double>(A) : adj_seed<float>(A);
+    if (rc) return rc;
+    if ((rc = adj_restore(A, sl))) return rc;
+    if ((rc = adj_advance(A, k - lo))) return rc;
+  }
+  return adj_step_any(A);
+}
+
+// Reverse steps [lo, hi) with S_lo in slot `sl` and the free slots in `pool`
+// (binomial checkpointing: with s free slots and t repetitions, C(s+t, s)
+// steps are reversible; the split leaves a tail reversible with one slot
+// fewer and a head with one repetition fewer).  With at least hi-lo-1 free
+// slots every state is stored once and no step is recomputed.
+static int adj_reverse(gp_adjoint *A, long lo, long hi, int sl, std::vector<int> &pool) {
+  int rc;
+  if (hi - lo == 1) return adj_one(A, lo, lo, sl);
+  if (pool.empty()) {
+    for (long k = hi - 1; k >= lo; k--)
+      if ((rc = adj_one(A, lo, k, sl))) return rc;
+    return GP_OK;
+  }
+  const long n = hi - lo;
+  const int nfree = (int)pool.size();
+  int t = 0;
+  while (binom(nfree, t) < n) t++;
+  long tail = std::min<long>(binom(nfree - 1, t), n - 1);
+  if (tail < 1) tail = 1;
+  const long m = hi - tail;
+  if ((rc = adj_restore(A, sl))) return rc;
+  if ((rc = adj_advance(A, m - lo))) return rc;
+  const int ns = pool.back();
+  pool.pop_back();
+  if ((rc = adj_save(A, ns))) return rc;
+  rc = adj_reverse(A, m, hi, ns, pool);
+  pool.push_back(ns);
+  if (rc) return rc;
+  return adj_reverse(A, lo, m, sl, pool);
+}
+
+extern "C" {
+
+int gp_adjoint_create(gp_ctx *c, long nsteps, int nsnap, gp_adjoint **out) {
+  if (!c || !out || nsteps < 1 || nsnap < 0 || c->nmember) return GP_E_ARG;
+  *out = nullptr;
+  if (c->P != 2 || c->C != 1 || c->hp.mu_solver != GP_MU_BINARY) return GP_E_UNSUPPORTED;
+  gp_adjoint *A = new (std::nothrow) gp_adjoint();
+  if (!A) return GP_E_NOMEM;
+  A->c = c;
+  A->nsteps = nsteps;
+  A->nslots = 1 + (int)std::min<long>(nsnap ? nsnap : nsteps, nsteps);
+  const size_t pb = (size_t)c->plane * c->esize;
+  A->slot_bytes = pb * (c->P + c->C + (c->mulo[0] ? c->C : 0));
+  bool ok = true;
+  A->slots.assign(A->nslots, nullptr);
+  for (int i = 0; i < A->nslots && ok; i++) ok = hipMalloc(&A->slots[i], A->slot_bytes) == hipSuccess;
+  ok = ok && hipMalloc(&A->cg, pb * gpk::CG_N) == hipSuccess && hipMalloc(&A->cf, pb * gpk::CF_N) == hipSuccess &&
+       hipMalloc(&A->lt, pb * 3) == hipSuccess;
+  for (int b = 0; b < 2 && ok; b++)
+    ok = hipMalloc(&A->lam[b][0], pb * 2) == hipSuccess && hipMalloc(&A->lam[b][1], pb) == hipSuccess &&
+         hipMemset(A->lam[b][0], 0, pb * 2) == hipSuccess && hipMemset(A->lam[b][1], 0, pb) == hipSuccess;
+  const int gx = (c->ry - 2 + gpk::ABX - 1) / gpk::ABX, gy = (c->rx - 2 + gpk::ABY - 1) / gpk::ABY;
+  A->nblk = gx * gy;
+  ok = ok && hipMalloc((void **)&A->part, sizeof(double) * gpk::ADJ_NPAR * A->nblk) == hipSuccess &&
+       hipMalloc((void **)&A->acc, sizeof(double) * gpk::ADJ_NPAR) == hipSuccess;
+  ok = ok && hipMemset(A->lt, 0, pb * 3) == hipSuccess;
+  ok = ok && hipEventCreate(&A->e0) == hipSuccess && hipEventCreate(&A->e1) == hipSuccess;
+  if (!ok) {
+    gp_adjoint_destroy(A);
+    return GP_E_NOMEM;
+  }
+  *out = A;
+  return GP_OK;
+}
+
+void gp_adjoint_destroy(gp_adjoint *A) {
+  if (!A) return;
+  if (A->c) (void)hipStreamSynchronize(A->c->stream);
+  for (void *s : A->slots)
+    if (s) (void)hipFree(s);
+  for (void *p : {A->cg, A->cf, A->lt, A->lam[0][0], A->lam[0][1], A->lam[1][0], A->lam[1][1]})
+    if (p) (void)hipFree(p);
+  if (A->part) (void)hipFree(A->part);
+  if (A->acc) (void)hipFree(A->acc);
+  if (A->e0) (void)hipEventDestroy(A->e0);
+  if (A->e1) (void)hipEventDestroy(A->e1);
+  delete A;
+}
+
+int gp_adjoint_run(gp_adjoint *A, const double *phi0, const double *mu0, const double *phi_target, double *loss,
+                   double *grad, double *lam_phi0, double *lam_mu0) {
+  if (!A || !phi0 || !mu0 || !phi_target) return GP_E_ARG;
+  gp_ctx *c = A->c;
+  int rc = gp_set_state(c, phi0, mu0);
+  if (rc) return rc;
+  A->fwd_ms = A->rev_ms = 0;
+  A->fwd_steps = 0;
+  A->seeded = false;
+  A->target = phi_target;
+  GP_HIP_CHECK(hipMemset(A->acc, 0, sizeof(double) * gpk::ADJ_NPAR));
+  if ((rc = adj_save(A, 0))) return rc;
+  std::vector<int> pool;
+  for (int i = A->nslots - 1; i >= 1; i--) pool.push_back(i);
+  rc = adj_reverse(A, 0, A->nsteps, 0, pool);
+  if (rc) return rc;
+  GP_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (loss) *loss = A->loss;
+  if (grad) GP_HIP_CHECK(hipMemcpy(grad, A->acc, sizeof(double) * gpk::ADJ_NPAR, hipMemcpyDeviceToHost));
+  if (lam_phi0 || lam_mu0) {
+    const long MX = c->hp.mesh_x, MY = c->hp.mesh_y;
+    const size_t pb = (size_t)c->plane * c->esize;
+    std::vector<char> hp(pb * 2), hm(pb);
+    GP_HIP_CHECK(hipMemcpy(hp.data(), A->lam[A->lcur][0], pb * 2, hipMemcpyDeviceToHost));
+    GP_HIP_CHECK(hipMemcpy(hm.data(), A->lam[A->lcur][1], pb, hipMemcpyDeviceToHost));
+    for (int a = 0; a < 2; a++)
+      for (long x = 0; x < MX; x++)
+        for (long y = 0; y < MY; y++) {
+          const size_t o = (size_t)a * c->plane + (size_t)(x + 3) * c->pitch + (y + 3);
+          const double vp = c->dtype == GP_F64 ? ((const double *)hp.data())[o] : ((const float *)hp.data())[o];
+          if (lam_phi0) lam_phi0[((size_t)a * MX + x) * MY + y] = vp;
+          if (a == 0 && lam_mu0) {
+            const size_t om = (size_t)(x + 3) * c->pitch + (y + 3);
+            lam_mu0[(size_t)x * MY + y] =
+                c->dtype == GP_F64 ? ((const double *)hm.data())[om] : ((const float *)hm.data())[om];
+          }
+        }
+  }
+  return GP_OK;
+}
+
+int gp_adjoint_info(gp_adjoint *A, double *fwd_ms, double *rev_ms, long *forward_steps) {
+  if (!A) return GP_E_ARG;
+  if (fwd_ms) *fwd_ms = A->fwd_ms;
+  if (rev_ms) *rev_ms = A->rev_ms;
+  if (forward_steps) *forward_steps = A->fwd_steps;
+  return GP_OK;
+}
+
+}  // extern "C"

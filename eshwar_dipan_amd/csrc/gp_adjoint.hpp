@@ -1,0 +1,764 @@
+// gp_adjoint.hpp -- reverse of one explicit step for two phases (solid 0,
+// liquid 1) and one independent component (BINARY), isothermal, obstacle or
+// well potential, four-fold anisotropy with rotation.
+//
+// The step is S_{n+1} = B(G(S_n, F(S_n))): F is the delta-phi of every cell of
+// the interior plus its ghost ring (solverloop_phasefield), G the mu and phi
+// update of the interior cells (solverloop_concentration), B the ghost copies.
+// Its adjoint is applied as four gathers, so no cell is written by two threads:
+//
+//   k_adj_conc   per interior cell c: reverse of G_c seeded with the adjoint of
+//                its new (phi, mu); writes the partials of G_c times the seed
+//                for every input of c's stencil into per-direction planes;
+//   k_adj_phase  per ring/interior cell i: gathers the adjoint of delta-phi(i)
+//                from the (up to) five G-stencils that read it, then reverses
+//                F_i; writes its stencil partials likewise;
+//   k_adj_gather per cell: sums the stencil partials that point at it;
+//   k_adj_fold   per interior cell: adds the adjoints of its ghost images
+//                (the reverse of B).
+//
+// Each cell program recomputes its forward values from S_n and delta-phi(S_n)
+// and differentiates the mathematical expressions of the forward kernels
+// (gp_cell.hpp); the piecewise parts (active set, simplex projection, bulk test)
+// are differentiated on the branch the forward took.  Parameter partials are
+// reduced per block in binary64.
+#pragma once
+
+#include "gp_kernels.hpp"
+
+namespace gpk {
+
+// parameters of the gradient: anisotropy strength dab[0][1], pair relaxation
+// constant Tau[0][1], interface energy GAMMA[0][1], liquid diffusivity.
+enum { ADJ_DAB = 0, ADJ_TAU = 1, ADJ_GAMMA = 2, ADJ_DL = 3, ADJ_NPAR = 4 };
+
+// stencil directions of a 3x3 block: dir = (i+1)*3 + (j+1) for offsets (i, j)
+__device__ __forceinline__ int dir9(int i, int j) { return (i + 1) * 3 + (j + 1); }
+// five-point directions: 0 centre, 1 +x, 2 -x, 3 +y, 4 -y
+constexpr int D5X[5] = {0, 1, -1, 0, 0};
+constexpr int D5Y[5] = {0, 0, 0, 1, -1};
+
+// a_c(q) = 1 - d (3 - 4 Q(q)), Q = (qx^4 + qy^4) / |q|^4, with its gradient
+// and Hessian in q (per unit d in g1/H1).
+template <class R>
+struct AcEval {
+  bool nz;
+  R ac, aq;      // a_c and d a_c / d dab = -(3 - 4Q)
+  R g[2], g1[2]; // grad a_c = dab * g1
+  R H[2][2];     // Hessian of a_c
+  R q2;
+};
+
+template <class R>
+__device__ __forceinline__ void ac_eval(R dab, const R* q, AcEval<R>& e) {
+  const R qx = q[0], qy = q[1];
+  const R q2 = qx * qx + qy * qy;
+  e.q2 = q2;
+  e.nz = fabs(q2) > R(1.0e-15);
+  if (!e.nz) {
+    e.ac = R(1);
+    e.aq = R(0);
+    e.g[0] = e.g[1] = e.g1[0] = e.g1[1] = R(0);
+    e.H[0][0] = e.H[0][1] = e.H[1][0] = e.H[1][1] = R(0);
+    return;
+  }
+  const R q4 = qx * qx * qx * qx + qy * qy * qy * qy;
+  const R i2 = R(1) / q2, i4 = i2 * i2, i6 = i4 * i2, i8 = i4 * i4;
+  const R Q = q4 * i4;
+  e.aq = -(R(3) - R(4) * Q);
+  e.ac = R(1) + dab * e.aq;
+  e.g1[0] = R(16) * (qx * qx * qx * i4 - qx * q4 * i6);
+  e.g1[1] = R(16) * (qy * qy * qy * i4 - qy * q4 * i6);
+  e.g[0] = dab * e.g1[0];
+  e.g[1] = dab * e.g1[1];
+  const R hxx = R(12) * qx * qx * i4 - R(32) * qx * qx * qx * qx * i6 - R(4) * q4 * i6 + R(24) * qx * qx * q4 * i8;
+  const R hyy = R(12) * qy * qy * i4 - R(32) * qy * qy * qy * qy * i6 - R(4) * q4 * i6 + R(24) * qy * qy * q4 * i8;
+  const R hxy = -R(16) * qx * qx * qx * qy * i6 - R(16) * qx * qy * qy * qy * i6 + R(24) * qx * qy * q4 * i8;
+  e.H[0][0] = R(4) * dab * hxx;
+  e.H[1][1] = R(4) * dab * hyy;
+  e.H[0][1] = e.H[1][0] = R(4) * dab * hxy;
+}
+
+template <class R>
+__device__ __forceinline__ void mat2v(const R (&M)[2][2], const R* x, R* y) {
+  y[0] = M[0][0] * x[0] + M[0][1] * x[1];
+  y[1] = M[1][0] * x[0] + M[1][1] * x[1];
+}
+template <class R>
+__device__ __forceinline__ void mat2tv(const R (&M)[2][2], const R* x, R* y) {
+  y[0] = M[0][0] * x[0] + M[1][0] * x[1];
+  y[1] = M[0][1] * x[0] + M[1][1] * x[1];
+}
+
+// Face term of div(dA/d grad phi): E = ac^2 flux + ac |q|^2 (Minv grad a_c)_k,
+// q = M u.  Returns E and adds Ebar * dE/d(u, flux, dab) to ub, fb, dabb.
+template <class R>
+__device__ __forceinline__ R face_E_rev(const R (&M)[2][2], const R (&Mi)[2][2], R dab, const R* u, R flux, int k,
+                                        R Ebar, R* ub, R& fb, double& dabb) {
+  R q[2];
+  mat2v(M, u, q);
+  AcEval<R> e;
+  ac_eval(dab, q, e);
+  R rd[2], rd1[2];
+  mat2v(Mi, e.g, rd);
+  mat2v(Mi, e.g1, rd1);
+  const R E = e.ac * e.ac * flux + e.ac * e.q2 * rd[k];
+  if (Ebar == R(0)) return E;
+  fb += Ebar * e.ac * e.ac;
+  if (!e.nz) return E;
+  // dE/dq = 2 ac flux g + (q2 rd_k) g + ac rd_k 2q + ac q2 (Mi H)_k
+  R MiH[2];
+  MiH[0] = Mi[k][0] * e.H[0][0] + Mi[k][1] * e.H[1][0];
+  MiH[1] = Mi[k][0] * e.H[0][1] + Mi[k][1] * e.H[1][1];
+  R dq[2];
+  for (int i = 0; i < 2; i++)
+    dq[i] = Ebar * (R(2) * e.ac * flux * e.g[i] + e.q2 * rd[k] * e.g[i] + e.ac * rd[k] * R(2) * q[i] + e.ac * e.q2 * MiH[i]);
+  R du[2];
+  mat2tv(M, dq, du);
+  ub[0] += du[0];
+  ub[1] += du[1];
+  dabb += (double)(Ebar * (R(2) * e.ac * e.aq * flux + e.aq * e.q2 * rd[k] + e.ac * e.q2 * rd1[k]));
+  return E;
+}
+
+// Term of dA/dphi: E2 = ac^2 (q . r) + ac |q|^2 (grad a_c . r), q = M u,
+// r = M w.  Adds E2bar * dE2/d(u, w, dab).
+template <class R>
+__device__ __forceinline__ R dA_E2_rev(const R (&M)[2][2], R dab, const R* u, const R* w, R Ebar, R* ub, R* wb,
+                                       double& dabb) {
+  R q[2], r[2];
+  mat2v(M, u, q);
+  mat2v(M, w, r);
+  AcEval<R> e;
+  ac_eval(dab, q, e);
+  const R qr = q[0] * r[0] + q[1] * r[1];
+  const R gr = e.g[0] * r[0] + e.g[1] * r[1];
+  const R g1r = e.g1[0] * r[0] + e.g1[1] * r[1];
+  const R E2 = e.ac * e.ac * qr + e.ac * e.q2 * gr;
+  if (Ebar == R(0)) return E2;
+  R dr[2], dq[2];
+  for (int i = 0; i < 2; i++) dr[i] = Ebar * (e.ac * e.ac * q[i] + e.ac * e.q2 * e.g[i]);
+  if (e.nz) {
+    R Hr[2];
+    Hr[0] = e.H[0][0] * r[0] + e.H[0][1] * r[1];
+    Hr[1] = e.H[1][0] * r[0] + e.H[1][1] * r[1];
+    for (int i = 0; i < 2; i++)
+      dq[i] = Ebar * (R(2) * e.ac * e.g[i] * qr + e.ac * e.ac * r[i] + gr * (e.g[i] * e.q2 + e.ac * R(2) * q[i]) +
+                      e.ac * e.q2 * Hr[i]);
+    dabb += (double)(Ebar * (R(2) * e.ac * e.aq * qr + e.aq * e.q2 * gr + e.ac * e.q2 * g1r));
+  } else {
+    for (int i = 0; i < 2; i++) dq[i] = Ebar * (e.ac * e.ac * r[i]);
+  }
+  R t[2];
+  mat2tv(M, dq, t);
+  ub[0] += t[0];
+  ub[1] += t[1];
+  mat2tv(M, dr, t);
+  wb[0] += t[0];
+  wb[1] += t[1];
+  return E2;
+}
+
+// stencil quantities of one phase from its 3x3 block (make_nbhd) and their
+// adjoints; v[i][j] with i, j = 0..2 for offsets -1..+1
+template <class R>
+struct Stn {
+  R ph, gXc, gXb, sXc, sXb, gYc, gYl, sYc, sYl, cXc, cYc, cYf, cYb, cXr, cXl;
+};
+
+template <class R>
+__device__ __forceinline__ void stn_fwd(const DevParams<R>& p, const R (&v)[3][3], Stn<R>& s) {
+  s.ph = v[1][1];
+  s.gXc = (v[2][1] - v[1][1]) * p.inv_dx;
+  s.gXb = (v[1][1] - v[0][1]) * p.inv_dx;
+  s.sXc = R(0.5) * (v[2][1] + v[1][1]);
+  s.sXb = R(0.5) * (v[1][1] + v[0][1]);
+  s.gYc = (v[1][2] - v[1][1]) * p.inv_dy;
+  s.gYl = (v[1][1] - v[1][0]) * p.inv_dy;
+  s.sYc = R(0.5) * (v[1][2] + v[1][1]);
+  s.sYl = R(0.5) * (v[1][1] + v[1][0]);
+  s.cXc = R(0.5) * (s.gXc + s.gXb);
+  s.cYc = (v[1][2] - v[1][0]) * p.inv_two_dy;
+  s.cYf = (v[2][2] - v[2][0]) * p.inv_two_dy;
+  s.cYb = (v[0][2] - v[0][0]) * p.inv_two_dy;
+  s.cXr = R(0.5) * ((v[2][2] - v[1][2]) * p.inv_dx + (v[1][2] - v[0][2]) * p.inv_dx);
+  s.cXl = R(0.5) * ((v[2][0] - v[1][0]) * p.inv_dx + (v[1][0] - v[0][0]) * p.inv_dx);
+}
+
+template <class R>
+__device__ __forceinline__ void stn_rev(const DevParams<R>& p, Stn<R> b, R (&vb)[3][3]) {
+  b.gXc += R(0.5) * b.cXc;
+  b.gXb += R(0.5) * b.cXc;
+  vb[1][1] += b.ph;
+  vb[2][1] += b.gXc * p.inv_dx;
+  vb[1][1] -= b.gXc * p.inv_dx;
+  vb[1][1] += b.gXb * p.inv_dx;
+  vb[0][1] -= b.gXb * p.inv_dx;
+  vb[2][1] += R(0.5) * b.sXc;
+  vb[1][1] += R(0.5) * b.sXc;
+  vb[1][1] += R(0.5) * b.sXb;
+  vb[0][1] += R(0.5) * b.sXb;
+  vb[1][2] += b.gYc * p.inv_dy;
+  vb[1][1] -= b.gYc * p.inv_dy;
+  vb[1][1] += b.gYl * p.inv_dy;
+  vb[1][0] -= b.gYl * p.inv_dy;
+  vb[1][2] += R(0.5) * b.sYc;
+  vb[1][1] += R(0.5) * b.sYc;
+  vb[1][1] += R(0.5) * b.sYl;
+  vb[1][0] += R(0.5) * b.sYl;
+  vb[1][2] += b.cYc * p.inv_two_dy;
+  vb[1][0] -= b.cYc * p.inv_two_dy;
+  vb[2][2] += b.cYf * p.inv_two_dy;
+  vb[2][0] -= b.cYf * p.inv_two_dy;
+  vb[0][2] += b.cYb * p.inv_two_dy;
+  vb[0][0] -= b.cYb * p.inv_two_dy;
+  const R hx = R(0.5) * p.inv_dx;
+  vb[2][2] += b.cXr * hx;
+  vb[0][2] -= b.cXr * hx;
+  vb[2][0] += b.cXl * hx;
+  vb[0][0] -= b.cXl * hx;
+}
+
+template <class R>
+__device__ __forceinline__ void stn_zero(Stn<R>& b) {
+  b.ph = b.gXc = b.gXb = b.sXc = b.sXb = b.gYc = b.gYl = b.sYc = b.sYl = R(0);
+  b.cXc = b.cYc = b.cYf = b.cYb = b.cXr = b.cXl = R(0);
+}
+
+// psi_a(mu) and d psi_a / d mu (C = 1)
+template <class R>
+__device__ __forceinline__ R psi1(const DevParams<R>& p, R mu, int a, R& dpsi) {
+  const R c = p.cmu[a][0][0] * (mu - p.Bsh[a][0]);
+  const R dc = p.cmu[a][0][0];
+  dpsi = -c - mu * dc + (R(2) * p.A[a][0][0] * c + p.B[a][0]) * dc;
+  return -mu * c + p.A[a][0][0] * c * c + p.B[a][0] * c + p.Cf[a];
+}
+
+// ---------------------------------------------------------------- phase sweep
+
+// Reverse of delta-phi of one cell.  v: phi 3x3 blocks, mu: potential of the
+// cell, seed: adjoint of delta-phi.  Adds to vb (3x3 phi adjoints), mub, par.
+template <class R>
+__device__ __forceinline__ void adj_phase_cell(const DevParams<R>& p, const R (&v)[2][3][3], R mu, const R* seed,
+                                               R (&vb)[2][3][3], R& mub, double* par) {
+  Stn<R> s[2], sb[2];
+  stn_fwd(p, v[0], s[0]);
+  stn_fwd(p, v[1], s[1]);
+  stn_zero(sb[0]);
+  stn_zero(sb[1]);
+  R ph[2] = {s[0].ph, s[1].ph}, phb[2] = {R(0), R(0)};
+  bool act[2];
+  for (int a = 0; a < 2; a++) {
+    R d = (s[a].gXc - s[a].gXb) * p.inv_dx + (s[a].gYc - s[a].gYl) * p.inv_dy;
+    act[a] = fabs(d) > R(0);
+  }
+  const bool any = act[0] || act[1];
+  // the isotropic forms are the anisotropic ones at dab = 0 with identity rotations
+  const bool an = p.anisotropy != 0;
+  const R gam = p.gamma[0][1], dab = an ? p.dab[0][1] : R(0);
+  R I2[2][2] = {{R(1), R(0)}, {R(0), R(1)}};
+  auto Mrot = [&](int a, int b) -> const R(&)[2][2] {
+    if (an) return p.rot[a][b];
+    return I2;
+  };
+  auto Miv = [&](int a, int b) -> const R(&)[2][2] {
+    if (an) return p.rot_inv[a][b];
+    return I2;
+  };
+  // ---- forward values
+  R lam[2] = {R(0), R(0)}, psi[2], dpsi[2];
+  R tau = p.tau;
+  const bool pairtau = ph[0] * ph[1] != R(0);
+  if (any) {
+    psi[0] = psi1(p, mu, 0, dpsi[0]);
+    psi[1] = psi1(p, mu, 1, dpsi[1]);
+    if (pairtau) tau = p.tau_ab[0][1];
+  }
+  R dp0[2] = {R(0), R(0)};
+  int nact = (int)act[0] + (int)act[1];
+  // the forward lambda is needed for dp0 (projection branches); recompute it
+  // with the same expressions as the reverse below
+  auto lam_of = [&](int a) -> R {
+    const int b = 1 - a;
+    const Stn<R>&A = s[a], &B = s[b];
+    R u[2] = {A.ph * B.cXc - B.ph * A.cXc, A.ph * B.cYc - B.ph * A.cYc};
+    R w[2] = {B.cXc, B.cYc};
+    R ub[2] = {R(0), R(0)}, wb[2] = {R(0), R(0)};
+    double dd = 0;
+    R dA = R(2) * gam * dA_E2_rev(Mrot(a, b), dab, u, w, R(0), ub, wb, dd);
+    R fb = R(0);
+    R uF[2] = {A.sXc * B.gXc - B.sXc * A.gXc, R(0.5) * (A.sXc * (B.cYc + B.cYf) - B.sXc * (A.cYc + A.cYf))};
+    R uB[2] = {A.sXb * B.gXb - B.sXb * A.gXb, R(0.5) * (A.sXb * (B.cYb + B.cYc) - B.sXb * (A.cYb + A.cYc))};
+    R uR[2] = {R(0.5) * (A.sYc * (B.cXc + B.cXr) - B.sYc * (A.cXc + A.cXr)), A.sYc * B.gYc - B.sYc * A.gYc};
+    R uL[2] = {R(0.5) * (A.sYl * (B.cXl + B.cXc) - B.sYl * (A.cXl + A.cXc)), A.sYl * B.gYl - B.sYl * A.gYl};
+    R EF = face_E_rev(Mrot(a, b), Miv(a, b), dab, uF, uF[0], 0, R(0), ub, fb, dd);
+    R EB = face_E_rev(Mrot(a, b), Miv(a, b), dab, uB, uB[0], 0, R(0), ub, fb, dd);
+    R ER = face_E_rev(Mrot(a, b), Miv(a, b), dab, uR, uR[1], 1, R(0), ub, fb, dd);
+    R EL = face_E_rev(Mrot(a, b), Miv(a, b), dab, uL, uL[1], 1, R(0), ub, fb, dd);
+    R S = ((-B.sXc) * EF - (-B.sXb) * EB) * p.inv_dx + ((-B.sYc) * ER - (-B.sYl) * EL) * p.inv_dy;
+    R div = R(2) * gam * S;
+    R dw = R(0);
+    if (act[b]) {
+      if (p.obstacle) dw = p.w_pref * ((A.ph * B.ph) >= R(0) ? gam * B.ph : -gam * B.ph);
+      else dw = p.w_pref * R(2) * gam * B.ph * B.ph * A.ph;
+    }
+    R dps = psi[a] * R(6) * A.ph * (R(1) - A.ph);
+    return p.epsilon * (-dA + div) - p.inv_eps * dw - dps;
+  };
+  if (any) {
+    R sl = R(0);
+    for (int a = 0; a < 2; a++)
+      if (act[a]) {
+        lam[a] = lam_of(a);
+        sl += lam[a];
+      }
+    sl /= R(nact);
+    for (int a = 0; a < 2; a++)
+      if (act[a]) dp0[a] = p.dt * (lam[a] - sl) / (tau * p.epsilon);
+  }
+  // ---- projection (project_simplex), forward with its branch record
+  R dp[2] = {dp0[0], dp0[1]};
+  bool inS[2], clip[2];
+  R sum_phib = R(0);
+  int count = 0;
+  for (int a = 0; a < 2; a++) {
+    inS[a] = act[a] && (ph[a] + dp[a]) > R(0) && (ph[a] + dp[a]) < R(1);
+    if (inS[a]) {
+      count++;
+      sum_phib += ph[a];
+    }
+  }
+  R Dphi = R(0);
+  for (int a = 0; a < 2; a++) {
+    clip[a] = (ph[a] + dp[a]) < R(0);
+    if (clip[a]) {
+      R Da = fabs(dp[a] + ph[a]);
+      dp[a] += Da;
+      Dphi += Da;
+    }
+  }
+  const bool redis = Dphi > R(0);
+  bool inS2[2] = {false, false};
+  const bool useSum = fabs(sum_phib) > R(0);
+  if (redis)
+    for (int b = 0; b < 2; b++) {
+      inS2[b] = act[b] && (ph[b] + dp[b]) > R(0) && (ph[b] + dp[b]) < R(1);
+      if (inS2[b]) {
+        if (useSum) dp[b] -= Dphi * ph[b] / sum_phib;
+        else dp[b] -= Dphi / R(count);
+      }
+    }
+  int top = -1;
+  for (int a = 0; a < 2; a++)
+    if (top < 0 && act[a] && (ph[a] + dp[a]) > R(1)) top = a;
+  // ---- reverse of the projection
+  R g2[2] = {seed[0], seed[1]};
+  if (top >= 0) {
+    for (int b = 0; b < 2; b++) {
+      if (b == top) {
+        phb[b] -= g2[b];
+        g2[b] = R(0);
+      } else if (act[b]) {
+        phb[b] -= g2[b];
+        g2[b] = R(0);
+      }
+    }
+  }
+  R g1[2] = {g2[0], g2[1]};
+  R Dphib = R(0);
+  if (redis) {
+    R sumb = R(0);
+    for (int b = 0; b < 2; b++)
+      if (inS2[b]) {
+        if (useSum) {
+          Dphib -= g2[b] * ph[b] / sum_phib;
+          phb[b] -= g2[b] * Dphi / sum_phib;
+          sumb += g2[b] * Dphi * ph[b] / (sum_phib * sum_phib);
+        } else {
+          Dphib -= g2[b] / R(count);
+        }
+      }
+    for (int a = 0; a < 2; a++)
+      if (inS[a]) phb[a] += sumb;
+  }
+  R g0[2];
+  for (int a = 0; a < 2; a++) {
+    if (clip[a]) {
+      // dp1 = -ph; Dphi += -(dp0 + ph)
+      phb[a] -= g1[a];
+      g0[a] = -Dphib;
+      phb[a] -= Dphib;
+    } else {
+      g0[a] = g1[a];
+    }
+  }
+  // ---- reverse of dp0 = dt (lam - mean) / (tau eps)
+  if (any) {
+    R mg = R(0);
+    for (int a = 0; a < 2; a++)
+      if (act[a]) mg += g0[a];
+    mg /= R(nact);
+    R tb = R(0);
+    for (int a = 0; a < 2; a++)
+      if (act[a]) tb -= g0[a] * dp0[a] / tau;
+    if (pairtau) par[ADJ_TAU] += (double)tb;
+    double gb = 0, db = 0;
+    for (int a = 0; a < 2; a++) {
+      if (!act[a]) continue;
+      const int b = 1 - a;
+      const R lb = p.dt * (g0[a] - mg) / (tau * p.epsilon);
+      if (lb == R(0)) continue;
+      Stn<R>&A = s[a], &B = s[b], &Ab = sb[a], &Bb = sb[b];
+      // lam = eps (-dA + div) - inv_eps dw - dps
+      const R dAb = -p.epsilon * lb, divb = p.epsilon * lb, dwb = -p.inv_eps * lb, dpsb = -lb;
+      // dps = psi_a 6 ph_a (1 - ph_a)
+      mub += dpsb * R(6) * A.ph * (R(1) - A.ph) * dpsi[a];
+      Ab.ph += dpsb * psi[a] * (R(6) - R(12) * A.ph);
+      // dw
+      if (act[b]) {
+        if (p.obstacle) {
+          const R sg = (A.ph * B.ph) >= R(0) ? R(1) : R(-1);
+          Bb.ph += dwb * p.w_pref * sg * gam;
+          gb += (double)(dwb * p.w_pref * sg * B.ph);
+        } else {
+          Bb.ph += dwb * p.w_pref * R(2) * gam * R(2) * B.ph * A.ph;
+          Ab.ph += dwb * p.w_pref * R(2) * gam * B.ph * B.ph;
+          gb += (double)(dwb * p.w_pref * R(2) * B.ph * B.ph * A.ph);
+        }
+      }
+      // dA = 2 gam E2(u, w)
+      {
+        R u[2] = {A.ph * B.cXc - B.ph * A.cXc, A.ph * B.cYc - B.ph * A.cYc};
+        R w[2] = {B.cXc, B.cYc};
+        R ub[2] = {R(0), R(0)}, wb[2] = {R(0), R(0)};
+        R E2 = dA_E2_rev(Mrot(a, b), dab, u, w, dAb * R(2) * gam, ub, wb, db);
+        gb += (double)(dAb * R(2) * E2);
+        Ab.ph += ub[0] * B.cXc + ub[1] * B.cYc;
+        Bb.cXc += ub[0] * A.ph + wb[0];
+        Bb.ph -= ub[0] * A.cXc + ub[1] * A.cYc;
+        Ab.cXc -= ub[0] * B.ph;
+        Bb.cYc += ub[1] * A.ph + wb[1];
+        Ab.cYc -= ub[1] * B.ph;
+      }
+      // div = 2 gam S, S = sum over faces of +-(-s) E / h
+      {
+        R uF[2] = {A.sXc * B.gXc - B.sXc * A.gXc, R(0.5) * (A.sXc * (B.cYc + B.cYf) - B.sXc * (A.cYc + A.cYf))};
+        R uB[2] = {A.sXb * B.gXb - B.sXb * A.gXb, R(0.5) * (A.sXb * (B.cYb + B.cYc) - B.sXb * (A.cYb + A.cYc))};
+        R uR[2] = {R(0.5) * (A.sYc * (B.cXc + B.cXr) - B.sYc * (A.cXc + A.cXr)), A.sYc * B.gYc - B.sYc * A.gYc};
+        R uL[2] = {R(0.5) * (A.sYl * (B.cXl + B.cXc) - B.sYl * (A.cXl + A.cXc)), A.sYl * B.gYl - B.sYl * A.gYl};
+        R d0[2] = {R(0), R(0)};
+        R fz = R(0);
+        double dz = 0;
+        const R EF = face_E_rev(Mrot(a, b), Miv(a, b), dab, uF, uF[0], 0, R(0), d0, fz, dz);
+        const R EB = face_E_rev(Mrot(a, b), Miv(a, b), dab, uB, uB[0], 0, R(0), d0, fz, dz);
+        const R ER = face_E_rev(Mrot(a, b), Miv(a, b), dab, uR, uR[1], 1, R(0), d0, fz, dz);
+        const R EL = face_E_rev(Mrot(a, b), Miv(a, b), dab, uL, uL[1], 1, R(0), d0, fz, dz);
+        const R S = ((-B.sXc) * EF - (-B.sXb) * EB) * p.inv_dx + ((-B.sYc) * ER - (-B.sYl) * EL) * p.inv_dy;
+        gb += (double)(divb * R(2) * S);
+        const R Sb = divb * R(2) * gam;
+        const R GF = Sb * p.inv_dx, GB = -Sb * p.inv_dx, GR = Sb * p.inv_dy, GL = -Sb * p.inv_dy;
+        Bb.sXc -= GF * EF;
+        Bb.sXb -= GB * EB;
+        Bb.sYc -= GR * ER;
+        Bb.sYl -= GL * EL;
+        R uFb[2] = {R(0), R(0)}, uBb[2] = {R(0), R(0)}, uRb[2] = {R(0), R(0)}, uLb[2] = {R(0), R(0)};
+        R fF = R(0), fB = R(0), fR = R(0), fL = R(0);
+        face_E_rev(Mrot(a, b), Miv(a, b), dab, uF, uF[0], 0, -B.sXc * GF, uFb, fF, db);
+        face_E_rev(Mrot(a, b), Miv(a, b), dab, uB, uB[0], 0, -B.sXb * GB, uBb, fB, db);
+        face_E_rev(Mrot(a, b), Miv(a, b), dab, uR, uR[1], 1, -B.sYc * GR, uRb, fR, db);
+        face_E_rev(Mrot(a, b), Miv(a, b), dab, uL, uL[1], 1, -B.sYl * GL, uLb, fL, db);
+        uFb[0] += fF;
+        uBb[0] += fB;
+        uRb[1] += fR;
+        uLb[1] += fL;
+        // uF
+        Ab.sXc += uFb[0] * B.gXc;
+        Bb.gXc += uFb[0] * A.sXc;
+        Bb.sXc -= uFb[0] * A.gXc;
+        Ab.gXc -= uFb[0] * B.sXc;
+        Ab.sXc += uFb[1] * R(0.5) * (B.cYc + B.cYf);
+        Bb.cYc += uFb[1] * R(0.5) * A.sXc;
+        Bb.cYf += uFb[1] * R(0.5) * A.sXc;
+        Bb.sXc -= uFb[1] * R(0.5) * (A.cYc + A.cYf);
+        Ab.cYc -= uFb[1] * R(0.5) * B.sXc;
+        Ab.cYf -= uFb[1] * R(0.5) * B.sXc;
+        // uB
+        Ab.sXb += uBb[0] * B.gXb;
+        Bb.gXb += uBb[0] * A.sXb;
+        Bb.sXb -= uBb[0] * A.gXb;
+        Ab.gXb -= uBb[0] * B.sXb;
+        Ab.sXb += uBb[1] * R(0.5) * (B.cYb + B.cYc);
+        Bb.cYb += uBb[1] * R(0.5) * A.sXb;
+        Bb.cYc += uBb[1] * R(0.5) * A.sXb;
+        Bb.sXb -= uBb[1] * R(0.5) * (A.cYb + A.cYc);
+        Ab.cYb -= uBb[1] * R(0.5) * B.sXb;
+        Ab.cYc -= uBb[1] * R(0.5) * B.sXb;
+        // uR
+        Ab.sYc += uRb[0] * R(0.5) * (B.cXc + B.cXr);
+        Bb.cXc += uRb[0] * R(0.5) * A.sYc;
+        Bb.cXr += uRb[0] * R(0.5) * A.sYc;
+        Bb.sYc -= uRb[0] * R(0.5) * (A.cXc + A.cXr);
+        Ab.cXc -= uRb[0] * R(0.5) * B.sYc;
+        Ab.cXr -= uRb[0] * R(0.5) * B.sYc;
+        Ab.sYc += uRb[1] * B.gYc;
+        Bb.gYc += uRb[1] * A.sYc;
+        Bb.sYc -= uRb[1] * A.gYc;
+        Ab.gYc -= uRb[1] * B.sYc;
+        // uL
+        Ab.sYl += uLb[0] * R(0.5) * (B.cXl + B.cXc);
+        Bb.cXl += uLb[0] * R(0.5) * A.sYl;
+        Bb.cXc += uLb[0] * R(0.5) * A.sYl;
+        Bb.sYl -= uLb[0] * R(0.5) * (A.cXl + A.cXc);
+        Ab.cXl -= uLb[0] * R(0.5) * B.sYl;
+        Ab.cXc -= uLb[0] * R(0.5) * B.sYl;
+        Ab.sYl += uLb[1] * B.gYl;
+        Bb.gYl += uLb[1] * A.sYl;
+        Bb.sYl -= uLb[1] * A.gYl;
+        Ab.gYl -= uLb[1] * B.sYl;
+      }
+    }
+    par[ADJ_GAMMA] += gb;
+    if (an) par[ADJ_DAB] += db;
+  }
+  sb[0].ph += phb[0];
+  sb[1].ph += phb[1];
+  stn_rev(p, sb[0], vb[0]);
+  stn_rev(p, sb[1], vb[1]);
+}
+
+// ---------------------------------------------------------------- mu sweep
+
+// s(phi) of the obstacle antitrapping for phase 0 of two, and its derivative
+template <class R>
+__device__ __forceinline__ R sphi0(R f, R& ds) {
+  const R q = f * (R(1) - f);
+  if (!(q > R(0))) {
+    ds = R(0);
+    return R(0);
+  }
+  const R h = R(3) * f * f - R(2) * f * f * f, hp = R(6) * f - R(6) * f * f;
+  const R sq = sqrt(q);
+  ds = ((R(1) - h) - f * hp) / sq - f * (R(1) - h) * (R(1) - R(2) * f) / (R(2) * q * sq);
+  return f * (R(1) - h) / sq;
+}
+
+// one antitrapping face: p = lower cell, q = upper cell; returns jat and adds
+// jb * d jat / d(inputs) to the adjoints
+template <class R>
+__device__ __forceinline__ R face_jat_rev(const DevParams<R>& p, R g0, R gL, R ctp0, R ctq0, R ctpL, R ctqL,
+                                          R php, R phq, R mup, R muq, R dpp, R dpq, bool enabled, R jb,
+                                          R& g0b, R& gLb, R& ctp0b, R& ctq0b, R& ctpLb, R& ctqLb,
+                                          R& phpb, R& phqb, R& mupb, R& muqb, R& dppb, R& dpqb, double& dlb) {
+  if (!enabled) return R(0);
+  const R tl = R(0.5) * (ctpL + ctqL);
+  const R ngl = sqrt_nz(tl * tl + gL * gL);
+  const R t = R(0.5) * (ctp0 + ctq0);
+  const R ngp = sqrt_nz(t * t + g0 * g0);
+  if (ngp == R(0)) return R(0);
+  R dsq = R(0), dsp = R(0), s_q = R(1), s_p = R(1);
+  if (p.obstacle) {
+    s_q = sphi0(phq, dsq);
+    s_p = sphi0(php, dsp);
+  }
+  const R N = g0 * gL + t * tl;
+  const bool nl = ngl > R(0);
+  const R sp = nl ? N / (ngl * ngp) : N;
+  const R cmd = p.cmu[1][0][0], cms = p.cmu[0][0][0];
+  const R dcq = cmd * (muq - p.Bsh[1][0]) - cms * (muq - p.Bsh[0][0]);
+  const R dcp = cmd * (mup - p.Bsh[1][0]) - cms * (mup - p.Bsh[0][0]);
+  const R pre = p.obstacle ? R(0.5) : p.well_at;
+  const R X = pre * (s_q * dcq * dpq + s_p * dcp * dpp);
+  const R A = p.at_coef[0][0];
+  const R asp = fabs(sp);
+  const R jat = A * asp * X * g0 / ngp;
+  if (jb == R(0)) return jat;
+  // at_coef = 1 - D_s / D_l
+  const R Dl = p.diff_l, Ds = p.diff_s;
+  dlb += (double)(jb * asp * X * g0 / ngp * (Ds / (Dl * Dl)));
+  const R spb = jb * A * (sp >= R(0) ? R(1) : R(-1)) * X * g0 / ngp;
+  const R Xb = jb * A * asp * g0 / ngp;
+  g0b += jb * A * asp * X / ngp;
+  R ngpb = -jb * A * asp * X * g0 / (ngp * ngp);
+  // X
+  if (p.obstacle) {
+    phqb += Xb * pre * dsq * dcq * dpq;
+    phpb += Xb * pre * dsp * dcp * dpp;
+  }
+  muqb += Xb * pre * s_q * dpq * (cmd - cms);
+  mupb += Xb * pre * s_p * dpp * (cmd - cms);
+  dpqb += Xb * pre * s_q * dcq;
+  dppb += Xb * pre * s_p * dcp;
+  // sp
+  R Nb, nglb = R(0);
+  if (nl) {
+    Nb = spb / (ngl * ngp);
+    nglb -= spb * N / (ngl * ngl * ngp);
+    ngpb -= spb * N / (ngl * ngp * ngp);
+  } else {
+    Nb = spb;
+  }
+  g0b += Nb * gL;
+  gLb += Nb * g0;
+  R tb = Nb * tl, tlb = Nb * t;
+  tb += ngpb * t / ngp;
+  g0b += ngpb * g0 / ngp;
+  if (nl) {
+    tlb += nglb * tl / ngl;
+    gLb += nglb * gL / ngl;
+  }
+  ctp0b += R(0.5) * tb;
+  ctq0b += R(0.5) * tb;
+  ctpLb += R(0.5) * tlb;
+  ctqLb += R(0.5) * tlb;
+  return jat;
+}
+
+// Reverse of the mu/phi update of one interior cell.  v: phi 3x3 blocks,
+// mu5/dp5: mu and delta-phi of the five-point stencil (D5X/D5Y order),
+// seedp/seedm: adjoints of the new phi and mu.  Adds to vb, mub5, dpb5, par.
+template <class R>
+__device__ __forceinline__ void adj_conc_cell(const DevParams<R>& p, const R (&v)[2][3][3], const R* mu5,
+                                              const R (&dp5)[5][2], const R* seedp, R seedm, int y,
+                                              R (&vb)[2][3][3], R* mub5, R (&dpb5)[5][2], double* par) {
+  Stn<R> s[2], sb[2];
+  stn_fwd(p, v[0], s[0]);
+  stn_fwd(p, v[1], s[1]);
+  stn_zero(sb[0]);
+  stn_zero(sb[1]);
+  // phi of the five cells
+  R ph5[5][2];
+  for (int d = 0; d < 5; d++)
+    for (int a = 0; a < 2; a++) ph5[d][a] = v[a][1 + D5X[d]][1 + D5Y[d]];
+  R phb5[5][2] = {};
+  // mobilities
+  const R dcm0 = p.dcm[0][0][0][0], dcm1 = p.dcm[1][0][0][0];
+  R D5[5];
+  for (int d = 0; d < 5; d++) D5[d] = dcm0 * ph5[d][0] + dcm1 * ph5[d][1];
+  const R Dmf = R(0.5) * (D5[1] + D5[0]), Dmb = R(0.5) * (D5[0] + D5[2]);
+  const R Dmr = R(0.5) * (D5[3] + D5[0]), Dml = R(0.5) * (D5[0] + D5[4]);
+  const R gmf = (mu5[1] - mu5[0]) * p.inv_dx, gmb = (mu5[0] - mu5[2]) * p.inv_dx;
+  const R gmr = (mu5[3] - mu5[0]) * p.inv_dy, gml = (mu5[0] - mu5[4]) * p.inv_dy;
+  const R divflux = (Dmf * gmf - Dmb * gmb) * p.inv_dx + (Dmr * gmr - Dml * gml) * p.inv_dy;
+  // antitrapping (forward values only here; reverse below with the seeds)
+  R z = R(0), zb;
+  double dz = 0;
+  auto J = [&](int face, R jb, R* out_b, double& dl) -> R {
+    // face 0: c|f, 1: b|c, 2: c|r, 3: l|c ; returns jat
+    R g0, gL, ctp0, ctq0, ctpL, ctqL;
+    int ip, iq;
+    bool en = true;
+    if (face == 0) { g0 = s[0].gXc; gL = s[1].gXc; ctp0 = s[0].cYc; ctq0 = s[0].cYf; ctpL = s[1].cYc; ctqL = s[1].cYf; ip = 0; iq = 1; }
+    else if (face == 1) { g0 = s[0].gXb; gL = s[1].gXb; ctp0 = s[0].cYb; ctq0 = s[0].cYc; ctpL = s[1].cYb; ctqL = s[1].cYc; ip = 2; iq = 0; }
+    else if (face == 2) { g0 = s[0].gYc; gL = s[1].gYc; ctp0 = s[0].cXc; ctq0 = s[0].cXr; ctpL = s[1].cXc; ctqL = s[1].cXr; ip = 0; iq = 3; en = y != p.yskip; }
+    else { g0 = s[0].gYl; gL = s[1].gYl; ctp0 = s[0].cXl; ctq0 = s[0].cXc; ctpL = s[1].cXl; ctqL = s[1].cXc; ip = 4; iq = 0; en = (y - 1) != p.yskip; }
+    // out_b: g0, gL, ctp0, ctq0, ctpL, ctqL, php, phq, mup, muq, dpp, dpq
+    return face_jat_rev(p, g0, gL, ctp0, ctq0, ctpL, ctqL, ph5[ip][0], ph5[iq][0], mu5[ip], mu5[iq], dp5[ip][0],
+                        dp5[iq][0], en, jb, out_b[0], out_b[1], out_b[2], out_b[3], out_b[4], out_b[5], out_b[6],
+                        out_b[7], out_b[8], out_b[9], out_b[10], out_b[11], dl);
+  };
+  (void)z;
+  (void)zb;
+  (void)dz;
+  R tmp[12];
+  double dl0 = 0;
+  const R jf = J(0, R(0), tmp, dl0), jbk = J(1, R(0), tmp, dl0), jr = J(2, R(0), tmp, dl0), jl = J(3, R(0), tmp, dl0);
+  const R divjat = (jf - jbk) * p.jscale * p.inv_dx + (jr - jl) * p.jscale * p.inv_dy;
+  const R* phc = ph5[0];
+  int bulk = -1;
+  for (int a = 1; a >= 0; a--)
+    if (phc[a] == R(1)) bulk = a;
+  // ---- reverse
+  R divfluxb, divjatb;
+  R mucb = R(0);
+  for (int a = 0; a < 2; a++) {
+    phb5[0][a] += seedp[a];
+    dpb5[0][a] += seedp[a];
+  }
+  if (bulk >= 0) {
+    const R cnb = seedm * R(2) * p.A[bulk][0][0];
+    mucb += cnb * p.cmu[bulk][0][0];
+    divfluxb = cnb * p.dt;
+    divjatb = -cnb;
+  } else {
+    R cc[2], h[2], hp[2];
+    for (int a = 0; a < 2; a++) {
+      cc[a] = p.cmu[a][0][0] * (mu5[0] - p.Bsh[a][0]);
+      h[a] = R(3) * phc[a] * phc[a] - R(2) * phc[a] * phc[a] * phc[a];
+      hp[a] = R(6) * phc[a] - R(6) * phc[a] * phc[a];
+    }
+    R sum = R(0), dcdmu = R(0);
+    for (int a = 0; a < 2; a++) {
+      sum += cc[a] * R(6) * phc[a] * (R(1) - phc[a]) * dp5[0][a];
+      dcdmu += p.cmu[a][0][0] * h[a];
+    }
+    const R dc = p.dt * divflux - divjat - sum;
+    mucb += seedm;
+    const R dcb = seedm / dcdmu;
+    const R dcdmub = -seedm * dc / (dcdmu * dcdmu);
+    for (int a = 0; a < 2; a++) phb5[0][a] += dcdmub * p.cmu[a][0][0] * hp[a];
+    divfluxb = p.dt * dcb;
+    divjatb = -dcb;
+    const R sumb = -dcb;
+    for (int a = 0; a < 2; a++) {
+      mucb += sumb * p.cmu[a][0][0] * R(6) * phc[a] * (R(1) - phc[a]) * dp5[0][a];
+      phb5[0][a] += sumb * cc[a] * (R(6) - R(12) * phc[a]) * dp5[0][a];
+      dpb5[0][a] += sumb * cc[a] * R(6) * phc[a] * (R(1) - phc[a]);
+    }
+  }
+  // divflux
+  {
+    const R a1 = divfluxb * p.inv_dx, a2 = divfluxb * p.inv_dy;
+    const R Dmfb = a1 * gmf, Dmbb = -a1 * gmb, Dmrb = a2 * gmr, Dmlb = -a2 * gml;
+    const R gmfb = a1 * Dmf, gmbb = -a1 * Dmb, gmrb = a2 * Dmr, gmlb = -a2 * Dml;
+    R Db[5];
+    Db[0] = R(0.5) * (Dmfb + Dmbb + Dmrb + Dmlb);
+    Db[1] = R(0.5) * Dmfb;
+    Db[2] = R(0.5) * Dmbb;
+    Db[3] = R(0.5) * Dmrb;
+    Db[4] = R(0.5) * Dmlb;
+    double dl = 0;
+    for (int d = 0; d < 5; d++) {
+      phb5[d][0] += Db[d] * dcm0;
+      phb5[d][1] += Db[d] * dcm1;
+      dl += (double)(Db[d] * p.cmu[1][0][0] * ph5[d][1]);
+    }
+    par[ADJ_DL] += dl;
+    mub5[1] += gmfb * p.inv_dx;
+    mub5[0] -= gmfb * p.inv_dx;
+    mub5[0] += gmbb * p.inv_dx;
+    mub5[2] -= gmbb * p.inv_dx;
+    mub5[3] += gmrb * p.inv_dy;
+    mub5[0] -= gmrb * p.inv_dy;
+    mub5[0] += gmlb * p.inv_dy;
+    mub5[4] -= gmlb * p.inv_dy;
+  }
+  mub5[0] += mucb;
+  // divjat
+  {
+    const R bx = divjatb * p.jscale * p.inv_dx, by = divjatb * p.jscale * p.inv_dy;
+    const R jbs[4] = {bx, -bx, by, -by};
+    double dl = 0;
+    for (int f = 0; f < 4; f++) {
+      R b[12] = {};
+      J(f, jbs[f], b, dl);
+      int ip, iq;
+      if (f == 0) { ip = 0; iq = 1; sb[0].gXc += b[0]; sb[1].gXc += b[1]; sb[0].cYc += b[2]; sb[0].cYf += b[3]; sb[1].cYc += b[4]; sb[1].cYf += b[5]; }
+      else if (f == 1) { ip = 2; iq = 0; sb[0].gXb += b[0]; sb[1].gXb += b[1]; sb[0].cYb += b[2]; sb[0].cYc += b[3]; sb[1].cYb += b[4]; sb[1].cYc += b[5]; }
+      else if (f == 2) { ip = 0; iq = 3; sb[0].gYc += b[0]; sb[1].gYc += b[1]; sb[0].cXc += b[2]; sb[0].cXr += b[3]; sb[1].cXc += b[4]; sb[1].cXr += b[5]; }
+      else { ip = 4; iq = 0; sb[0].gYl += b[0]; sb[1].gYl += b[1]; sb[0].cXl += b[2]; sb[0].cXc += b[3]; sb[1].cXl += b[4]; sb[1].cXc += b[5]; }
+      phb5[ip][0] += b[6];
+      phb5[iq][0] += b[7];
+      mub5[ip] += b[8];
+      mub5[iq] += b[9];
+      dpb5[ip][0] += b[10];
+      dpb5[iq][0] += b[11];
+    }
+    par[ADJ_DL] += dl;
+  }
+  stn_rev(p, sb[0], vb[0]);
+  stn_rev(p, sb[1], vb[1]);
+  for (int d = 0; d < 5; d++)
+    for (int a = 0; a < 2; a++) vb[a][1 + D5X[d]][1 + D5Y[d]] += phb5[d][a];
+}
+
+}  // namespace gpk

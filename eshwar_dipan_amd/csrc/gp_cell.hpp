@@ -36,6 +36,7 @@ struct DevParams {
   R w_pref;               // 16/(pi*pi) (obstacle) or 9.0 (well)
   R jscale;               // (-0.25*M_PI*epsilon) or (-1.0*epsilon)
   R well_at;              // 0.5/sqrt(2.0)
+  R diff_s, diff_l;       // DIFFUSIVITY of phase 0 and of the liquid, component 0 (adjoint)
   R gamma[PM][PM];
   R gamma_abc[PM][PM][PM];
   R tau_ab[PM][PM];

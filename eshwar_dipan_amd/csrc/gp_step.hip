@@ -75,6 +75,8 @@ static void fill_devparams(const gp_params &h, int rx, int ry, int pitch, long p
   d.w_pref = (R)(h.obstacle ? 16.0 / (M_PI * M_PI) : 9.0);
   d.jscale = (R)(h.obstacle ? (-0.25 * M_PI * h.epsilon) : (-1.0 * h.epsilon));
   d.well_at = (R)(0.5 / sqrt(2.0));
+  d.diff_s = (R)h.diffusivity[0][0][0];
+  d.diff_l = (R)h.diffusivity[h.nphases - 1][0][0];
   for (int a = 0; a < P; a++) {
     for (int b = 0; b < P; b++) {
       d.gamma[a][b] = (R)h.gamma[a][b];

@@ -71,6 +71,16 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = i
+        for name, args in [
+            ("gp_adjoint_create", [vp, l, i, ct.POINTER(vp)]),
+            ("gp_adjoint_run", [vp, vp, vp, vp, ct.POINTER(d), vp, vp, vp]),
+            ("gp_adjoint_info", [vp, ct.POINTER(d), ct.POINTER(d), pl]),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = i
+        L.gp_adjoint_destroy.argtypes = [vp]
+        L.gp_adjoint_destroy.restype = None
         L.gp_ensemble_destroy.argtypes = [vp]
         L.gp_ensemble_destroy.restype = None
         L.gp_destroy.argtypes = [vp]
@@ -325,3 +335,52 @@ class Ensemble:
 
     def synchronize(self):
         _check(lib().gp_ensemble_synchronize(self._h), "gp_ensemble_synchronize")
+
+
+class Adjoint:
+    """Gradients of 1/2 sum (phi_0(T) - target)^2 through `nsteps` steps of a
+    two-phase binary Solver (gp_adjoint_*), with `nsnap` snapshot slots of binomial
+    checkpointing (0: one per step)."""
+
+    PARAMS = ("dab", "tau", "gamma", "D_liquid")
+
+    def __init__(self, solver, nsteps, nsnap=0):
+        self.solver = solver
+        self.nsteps = int(nsteps)
+        h = ct.c_void_p()
+        _check(lib().gp_adjoint_create(solver._h, self.nsteps, int(nsnap), ct.byref(h)), "gp_adjoint_create")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().gp_adjoint_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, phi0, mu0, target, state_grad=False):
+        """Returns (loss, {param: dL/dparam}, (dL/dphi0, dL/dmu0) or None)."""
+        sp, sm = self.solver.shape
+        phi0 = np.ascontiguousarray(phi0, dtype=np.float64)
+        mu0 = np.ascontiguousarray(mu0, dtype=np.float64)
+        target = np.ascontiguousarray(target, dtype=np.float64)
+        if phi0.shape != sp or mu0.shape != sm or target.shape != sp[1:]:
+            raise ValueError("state or target shape mismatch")
+        loss = ct.c_double()
+        grad = np.zeros(4)
+        lp = np.zeros(sp) if state_grad else None
+        lm = np.zeros(sm) if state_grad else None
+        _check(lib().gp_adjoint_run(self._h, _ptr(phi0), _ptr(mu0), _ptr(target), ct.byref(loss), _ptr(grad),
+                                    _ptr(lp) if state_grad else None, _ptr(lm) if state_grad else None),
+               "gp_adjoint_run")
+        return loss.value, dict(zip(self.PARAMS, grad)), ((lp, lm) if state_grad else None)
+
+    def info(self):
+        """(forward ms, reverse ms, forward steps) of the last run."""
+        f, r, n = ct.c_double(), ct.c_double(), ct.c_long()
+        _check(lib().gp_adjoint_info(self._h, ct.byref(f), ct.byref(r), ct.byref(n)), "gp_adjoint_info")
+        return f.value, r.value, n.value

@@ -208,6 +208,28 @@ int  gp_ensemble_autotune(gp_ensemble *e, long steps, int *tile_x, int *tile_y);
 int  gp_ensemble_sched_info(gp_ensemble *e, long *ntiles, long *n_interface, long *n_bulk, int *tile_x, int *tile_y);
 int  gp_ensemble_synchronize(gp_ensemble *e);
 
+/*
+ * Adjoint: gradients of L = 1/2 sum over interior cells of (phi_0(T) - target)^2
+ * after T = nsteps steps of a context (two phases, one independent component,
+ * BINARY), with respect to the anisotropy strength dab[0][1], the pair
+ * relaxation constant Tau[0][1], the interface energy GAMMA[0][1] and the
+ * liquid diffusivity, and optionally to the initial state.  The forward steps
+ * run through the context's own kernels; the states are kept in nsnap snapshot
+ * slots of device memory with binomial checkpointing (nsnap = 0: one slot per
+ * step, no recomputation).  Parameter gradients are accumulated in binary64.
+ * No reference counterpart: the serial solver has no adjoint.
+ */
+typedef struct gp_adjoint gp_adjoint;
+int  gp_adjoint_create(gp_ctx *ctx, long nsteps, int nsnap, gp_adjoint **out);
+void gp_adjoint_destroy(gp_adjoint *adj);
+/* phi0/mu0: initial state (host layout above); target: P=0 plane (mesh_x*mesh_y);
+   grad[4] = dL/d(dab, Tau, GAMMA, D_liquid); lam_phi0/lam_mu0 (optional): dL/d(initial state). */
+int  gp_adjoint_run(gp_adjoint *adj, const double *phi0, const double *mu0, const double *target, double *loss,
+                    double *grad, double *lam_phi0, double *lam_mu0);
+/* device time of the forward (recomputation included) and reverse kernels of the
+   last run, and the number of forward steps taken */
+int  gp_adjoint_info(gp_adjoint *adj, double *fwd_ms, double *rev_ms, long *forward_steps);
+
 const char *gp_strerror(int code);
 
 #ifdef __cplusplus
