@@ -69,6 +69,85 @@ def precision_report(a):
     print(json.dumps(r), flush=True)
 
 
+def adjoint_report(a):
+    """Dendrite (2 phases, binary, 512^2, anisotropy 0.02): gradients of the phi
+    mismatch to a target made with perturbed (dab, Tau, GAMMA, D_liquid), by the
+    adjoint with all states kept and with 20 snapshots, against central
+    differences (h = 1e-6 relative); time of the adjoint run over the forward."""
+    import copy
+    import numpy as np
+    from eshwar_dipan_amd import gpstep, workloads
+    inp, phi0, mu0, nsteps_full, desc = workloads.WORKLOADS[a.workload or "c5"]()
+    steps = a.steps or nsteps_full
+    p = inp.p
+
+    def setp(q, name, v):
+        q = copy.deepcopy(q)
+        if name == "dab":
+            q.dab[0][1] = q.dab[1][0] = v
+        elif name == "tau":
+            q.tau_ab[0][1] = q.tau_ab[1][0] = v
+        elif name == "gamma":
+            q.gamma[0][1] = q.gamma[1][0] = v
+        else:
+            q.diffusivity[q.nphases - 1][0][0] = v
+        return q
+
+    def getp(q, name):
+        return {"dab": q.dab[0][1], "tau": q.tau_ab[0][1], "gamma": q.gamma[0][1],
+                "D_liquid": q.diffusivity[q.nphases - 1][0][0]}[name]
+
+    def forward(q, dtype="f64"):
+        s = gpstep.Solver(q, dtype)
+        s.set_state(phi0, mu0)
+        s.step(steps)
+        out = s.get_state()
+        s.close()
+        return out
+
+    q = p
+    for name, f in (("dab", 1.2), ("tau", 0.9), ("gamma", 1.05), ("D_liquid", 0.95)):
+        q = setp(q, name, getp(p, name) * f)
+    target = forward(q)[0][0]
+    out = {"metric": "adjoint / forward time; gradient error vs central differences", "steps": steps,
+           "config": {"workload": desc, "mesh_x": p.mesh_x, "mesh_y": p.mesh_y}, "runs": []}
+    for dtype in ("f64", "f32"):
+        s = gpstep.Solver(p, dtype)
+        s.set_state(phi0, mu0)
+        s.step(64)  # graph capture outside the timing
+        s.set_state(phi0, mu0)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        t_fwd = time.perf_counter() - t0
+        for nsnap in (0, 20):
+            adj = gpstep.Adjoint(s, steps, nsnap)
+            t0 = time.perf_counter()
+            loss, grad, _ = adj.run(phi0, mu0, target)
+            t_adj = time.perf_counter() - t0
+            f_ms, r_ms, nfwd = adj.info()
+            adj.close()
+            out["runs"].append({"dtype": dtype, "snapshots": nsnap or steps, "loss": loss, "grad": grad,
+                                "forward_s": t_fwd, "adjoint_s": t_adj, "ratio": t_adj / t_fwd,
+                                "adjoint_forward_steps": nfwd, "device_fwd_ms": f_ms, "device_rev_ms": r_ms})
+            print(f"{dtype} nsnap={nsnap}: adjoint/forward {t_adj / t_fwd:.2f}", file=sys.stderr, flush=True)
+        s.close()
+    fd = {}
+    for name in gpstep.Adjoint.PARAMS:
+        th = getp(p, name)
+        h = 1e-6 * abs(th)
+        lp = 0.5 * float(((forward(setp(p, name, th + h))[0][0] - target) ** 2).sum())
+        lm = 0.5 * float(((forward(setp(p, name, th - h))[0][0] - target) ** 2).sum())
+        fd[name] = (lp - lm) / (2 * h)
+    out["central_differences"] = fd
+    g = out["runs"][0]["grad"]
+    out["rel_err_vs_fd"] = {k: abs(g[k] - fd[k]) / max(abs(fd[k]), 1e-300) for k in fd}
+    g32 = out["runs"][2]["grad"]
+    out["f32_rel_err_vs_f64"] = {k: abs(g32[k] - g[k]) / max(abs(g[k]), 1e-300) for k in g}
+    print(json.dumps(out), flush=True)
+
+
 def bench_ensemble(a):
     """Ensemble workload: every member grid stepped together as one tile set."""
     import torch
@@ -148,12 +227,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy five times (calibration of the HBM byte counters)")
-    ap.add_argument("--report", choices=["precision"], default=None,
+    ap.add_argument("--report", choices=["precision", "adjoint"], default=None,
                     help="precision: binary32 against binary64 on --workload (default c3), per-field error, "
-                         "solute drift and time-to-solution ratio over --steps (default 2000)")
+                         "solute drift and time-to-solution ratio over --steps (default 2000); adjoint: gradients "
+                         "through the dendrite workload (default c5) against central differences, adjoint/forward time")
     a = ap.parse_args()
     if a.report == "precision":
         precision_report(a)
+        return
+    if a.report == "adjoint":
+        adjoint_report(a)
         return
     if a.copy_probe:
         import torch

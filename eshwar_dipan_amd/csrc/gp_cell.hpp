@@ -752,4 +752,171 @@ __device__ __forceinline__ void cell_update(const DevParams<R>& p, const R* ph, 
   for (int b = 0; b < P; b++) ph_o[b] = ph[b] + dp[b];
 }
 
+// ---------------------------------------------------------------- smoothing
+
+// Pre-loop smoothing sweep (NSMOOTH > 1): smooth() of solverloop/solverloop.h:72-90.
+// delta-phi without the driving force, isotropic gradient sums also for
+// anisotropic runs (function_A_01.h:73-99,220-262 return 2*(sum1 + sum2) with
+// sum2 = 0; function_A_00.h returns 2*sum), obstacle/well terms without sign
+// tests (functionW_01.h:41-63, functionW_02.h:35-58), unweighted projection
+// (simplex_projection.h:50-90).
+template <int P, class R>
+__device__ __forceinline__ R dwdphi_smooth(const DevParams<R>& p, const R* ph, const bool* act, int a) {
+  R sum = R(0);
+#pragma unroll
+  for (int b = 0; b < P; b++)
+    if (b != a && act[b]) {
+      if (p.obstacle) sum += p.gamma[a][b] * ph[b];
+      else sum += R(2.0) * p.gamma[a][b] * (ph[b] * ph[b] * ph[a]);
+    }
+  sum *= p.w_pref;
+#pragma unroll
+  for (int b = 0; b < P; b++)
+#pragma unroll
+    for (int c = 0; c < P; c++)
+      if (b != a && c != a && b < c && act[b] && act[c]) sum += p.gamma_abc[a][b][c] * ph[b] * ph[c];
+  return sum;
+}
+
+template <int P, class R>
+__device__ __forceinline__ void project_unweighted(const R* ph, const bool* act, R* dp) {
+  int count = 0;
+#pragma unroll
+  for (int a = 0; a < P; a++)
+    if (act[a] && (ph[a] + dp[a]) > R(0) && (ph[a] + dp[a]) < R(1.0)) count++;
+  bool done = false;
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    if (done || !act[a]) continue;
+    if ((ph[a] + dp[a]) > R(1.0)) {
+      dp[a] = (R(1.0) - ph[a]);
+#pragma unroll
+      for (int b = 0; b < P; b++)
+        if (act[b] && b != a) dp[b] = -ph[b];
+      done = true;
+      continue;
+    }
+    if ((ph[a] + dp[a]) < R(0)) {
+      R D = fabs(dp[a] + ph[a]);
+      dp[a] += D;
+#pragma unroll
+      for (int b = 0; b < P; b++)
+        if (act[b] && b != a && (ph[b] + dp[b]) > R(0) && (ph[b] + dp[b]) < R(1.0)) dp[b] -= D / R(count);
+    }
+  }
+}
+
+template <int P, int C, class R>
+__device__ __forceinline__ void cell_dphi_smooth(const DevParams<R>& p, const Nbhd<P, R>& n, R* dp) {
+  R divphi[P], lam[P];
+  bool act[P];
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    divphi[a] = div_dx(n.gXc[a] - n.gXb[a], p);
+    divphi[a] += div_dy(n.gYc[a] - n.gYl[a], p);
+    act[a] = fabs(divphi[a]) > R(0);
+    dp[a] = R(0);
+  }
+  R sl = R(0), active = R(0);
+#pragma unroll
+  for (int a = 0; a < P; a++)
+    if (act[a]) {
+      R dA = R(0), dv = R(0);
+#pragma unroll
+      for (int b = 0; b < P; b++) {
+        if (b == a || !n.present[b]) continue;  // an absent phase contributes exact zeros
+        R sp = R(0);
+        sp += (n.ph[a] * n.cXc[b] - n.ph[b] * n.cXc[a]) * n.cXc[b];
+        sp += (n.ph[a] * n.cYc[b] - n.ph[b] * n.cYc[a]) * n.cYc[b];
+        sp *= p.gamma[a][b];
+        dA += sp;
+        R fc = n.sXc[a] * n.gXc[b] - n.sXc[b] * n.gXc[a];
+        R fb = n.sXb[a] * n.gXb[b] - n.sXb[b] * n.gXb[a];
+        R d1 = div_dx(fc * (-n.sXc[b]) - (fb * (-n.sXb[b])), p);
+        R gc = n.sYc[a] * n.gYc[b] - n.sYc[b] * n.gYc[a];
+        R gl = n.sYl[a] * n.gYl[b] - n.sYl[b] * n.gYl[a];
+        d1 += div_dy(gc * (-n.sYc[b]) - (gl * (-n.sYl[b])), p);
+        d1 *= p.gamma[a][b];
+        dv += d1;
+      }
+      const R A = p.anisotropy ? R(2.0) * (dA + R(0)) : R(2.0) * dA;
+      const R D = p.anisotropy ? R(2.0) * (dv + R(0)) : R(2.0) * dv;
+      lam[a] = p.epsilon * (-A + D) - p.inv_eps * dwdphi_smooth<P>(p, n.ph, act, a);
+      sl += lam[a];
+      active += R(1);
+    }
+  if (active != R(0)) {
+    sl /= active;
+    R tau = tau_phi<P>(p, n.ph);
+#pragma unroll
+    for (int a = 0; a < P; a++)
+      if (act[a]) dp[a] = p.dt * (lam[a] - sl) / (tau * p.epsilon);
+  }
+  project_unweighted<P>(n.ph, act, dp);
+}
+
+// mu update of the smoothing sweep: diffusion alone, general LU inverse for any
+// solver flag, mu + delta_mu (calculate_divergence_concentration.h:154-253)
+template <int P, int C, class R>
+__device__ __forceinline__ void cell_update_smooth(const DevParams<R>& p, const R* ph, const R* mu,
+                                                   const R (&Dmf)[C][C], const R (&Dmb)[C][C],
+                                                   const R (&Dmr)[C][C], const R (&Dml)[C][C],
+                                                   const R* gmf, const R* gmb, const R* gmr, const R* gml,
+                                                   R* mu_o, R* dmu) {
+  R divflux[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    divflux[k] = R(0);
+#pragma unroll
+    for (int l = 0; l < C; l++) {
+      divflux[k] += div_dx(Dmf[k][l] * gmf[l] - Dmb[k][l] * gmb[l], p);
+      divflux[k] += div_dy(Dmr[k][l] * gmr[l] - Dml[k][l] * gml[l], p);
+    }
+  }
+  int bulk = -1;
+#pragma unroll
+  for (int a = P - 1; a >= 0; a--)
+    if (ph[a] == R(1.0)) bulk = a;
+  if (bulk >= 0) {
+    R cn[C];
+#pragma unroll
+    for (int a = 0; a < P; a++)
+      if (a == bulk) {
+#pragma unroll
+        for (int k = 0; k < C; k++) cn[k] = c_mu<C>(p, mu, a, k) + p.dt * (divflux[k]);
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+          mu_o[k] = Mu_of_c<C>(p, cn, a, k);
+          R d = R(0);
+#pragma unroll
+          for (int j = 0; j < C; j++) d += (j == k ? R(2.0) * p.A[a][k][k] : p.A[a][k][j]) * (p.dt * divflux[j]);
+          dmu[k] = d;
+        }
+      }
+  } else {
+    R dc[C], dcdmu[C][C], h[P], inv[C][C];
+#pragma unroll
+    for (int a = 0; a < P; a++) h[a] = hphi<P>(ph, a);
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      dc[k] = p.dt * (divflux[k]);
+#pragma unroll
+      for (int l = 0; l < C; l++) {
+        dcdmu[k][l] = R(0);
+#pragma unroll
+        for (int a = 0; a < P; a++) dcdmu[k][l] += p.cmu[a][k][l] * h[a];
+      }
+    }
+    lu_inverse<C>(dcdmu, inv);
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      R s = R(0);
+#pragma unroll
+      for (int l = 0; l < C; l++) s = s + inv[k][l] * dc[l];
+      dmu[k] = s;
+      mu_o[k] = s + mu[k];
+    }
+  }
+}
+
 }  // namespace gpk

@@ -67,6 +67,8 @@ template <int P, int C, class R>
 int run_steps(gp_ctx *c, long nsteps);
 template <int P, int C, class R>
 int run_stats(gp_ctx *c, gp_stats *st);
+template <int P, int C, class R>
+int run_smooth(gp_ctx *c, long nsweeps);
 
 }  // namespace gpk
 

@@ -319,4 +319,80 @@ __global__ void __launch_bounds__(256) k_stats(const DevParams<R>* __restrict__ 
   }
 }
 
+// One pre-loop smoothing sweep over the interior (smooth(), solverloop.h:72-90):
+// delta-phi of a cell enters only its own update, so each thread does both.
+template <int P, int C, class R>
+__global__ void __launch_bounds__(BX * BY) GP_HEAVY k_smooth(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                                    const R* __restrict__ mu, const R* __restrict__ mulo,
+                                                    R* __restrict__ phi_o, R* __restrict__ mu_o,
+                                                    R* __restrict__ mulo_o) {
+  const DevParams<R>& p = *prm;
+  const int y = 3 + blockIdx.x * BX + threadIdx.x;
+  const int x = 3 + blockIdx.y * BY + threadIdx.y;
+  if (x > p.mesh_x + 2 || y > p.mesh_y + 2) return;
+  R v[P][3][3];
+  load3x3<P>(p, phi, x, y, v);
+  Nbhd<P, R> n;
+  make_nbhd<P>(p, v, n);
+  R dp[P];
+  cell_dphi_smooth<P, C>(p, n, dp);
+  R phc[P], phf[P], phb[P], phr[P], phl[P];
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    phc[a] = v[a][1][1];
+    phf[a] = v[a][2][1];
+    phb[a] = v[a][0][1];
+    phr[a] = v[a][1][2];
+    phl[a] = v[a][1][0];
+  }
+  R mc[C], mf[C], mb[C], mr[C], ml[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    mc[k] = mu[idx(p, k, x, y)];
+    mf[k] = mu[idx(p, k, x + 1, y)];
+    mb[k] = mu[idx(p, k, x - 1, y)];
+    mr[k] = mu[idx(p, k, x, y + 1)];
+    ml[k] = mu[idx(p, k, x, y - 1)];
+  }
+  R Dc[C][C], Df[C][C], Db[C][C], Dr[C][C], Dl[C][C];
+  cell_D<P, C>(p, phc, Dc);
+  cell_D<P, C>(p, phf, Df);
+  cell_D<P, C>(p, phb, Db);
+  cell_D<P, C>(p, phr, Dr);
+  cell_D<P, C>(p, phl, Dl);
+  R Dmf[C][C], Dmb[C][C], Dmr[C][C], Dml[C][C], gmf[C], gmb[C], gmr[C], gml[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+#pragma unroll
+    for (int l = 0; l < C; l++) {
+      Dmf[k][l] = (Df[k][l] + Dc[k][l]) * R(0.5);
+      Dmb[k][l] = (Dc[k][l] + Db[k][l]) * R(0.5);
+      Dmr[k][l] = (Dr[k][l] + Dc[k][l]) * R(0.5);
+      Dml[k][l] = (Dc[k][l] + Dl[k][l]) * R(0.5);
+    }
+    gmf[k] = div_dx(mf[k] - mc[k], p);
+    gmb[k] = div_dx(mc[k] - mb[k], p);
+    gmr[k] = div_dy(mr[k] - mc[k], p);
+    gml[k] = div_dy(mc[k] - ml[k], p);
+  }
+  R muo[C], dmu[C];
+  cell_update_smooth<P, C>(p, phc, mc, Dmf, Dmb, Dmr, Dml, gmf, gmb, gmr, gml, muo, dmu);
+#pragma unroll
+  for (int a = 0; a < P; a++) write_with_ghosts(p, 0, phi_o + (size_t)a * p.plane, x, y, phc[a] + dp[a]);
+  if constexpr (Ops<R>::Exact) {
+#pragma unroll
+    for (int k = 0; k < C; k++) write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, muo[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      R lo = mulo[idx(p, k, x, y)];
+      R yk = dmu[k] - lo;
+      R t = mc[k] + yk;
+      R lo_n = (t - mc[k]) - yk;
+      write_with_ghosts(p, 1, mu_o + (size_t)k * p.plane, x, y, t);
+      mulo_o[idx(p, k, x, y)] = lo_n;
+    }
+  }
+}
+
 }  // namespace gpk

@@ -235,6 +235,22 @@ int run_steps(gp_ctx *c, long nsteps) {
 }
 
 template <int P, int C, class R>
+int run_smooth(gp_ctx *c, long nsweeps) {
+  const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
+  dim3 blk(BX, BY);
+  dim3 g((c->hp.mesh_y + BX - 1) / BX, (c->hp.mesh_x + BY - 1) / BY);
+  for (long t = 0; t < nsweeps; t++) {
+    hipLaunchKernelGGL((k_smooth<P, C, R>), g, blk, 0, c->stream, prm, (const R *)c->phi[c->cur],
+                       (const R *)c->mu[c->cur], (const R *)c->mulo[c->cur], (R *)c->phi[c->cur ^ 1],
+                       (R *)c->mu[c->cur ^ 1], (R *)c->mulo[c->cur ^ 1]);
+    c->cur ^= 1;
+  }
+  c->flags_valid = 0;
+  GP_HIP_CHECK(hipGetLastError());
+  return GP_OK;
+}
+
+template <int P, int C, class R>
 int run_stats(gp_ctx *c, gp_stats *st) {
   constexpr int NV = 3 * P + 3 * C + 1 + P + C;
   const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
@@ -272,4 +288,5 @@ int run_stats(gp_ctx *c, gp_stats *st) {
 
 #define GP_INSTANTIATE(P, C, R)                             \
   template int gpk::run_steps<P, C, R>(gp_ctx *, long);    \
+  template int gpk::run_smooth<P, C, R>(gp_ctx *, long);   \
   template int gpk::run_stats<P, C, R>(gp_ctx *, gp_stats *);

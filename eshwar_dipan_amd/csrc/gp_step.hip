@@ -440,6 +440,19 @@ int gp_autotune(gp_ctx *c, long steps, int *path, int *shape) {
   return GP_OK;
 }
 
+int gp_smooth(gp_ctx *c, long nsweeps) {
+  if (!c || nsweeps < 0 || c->nmember) return GP_E_ARG;
+  int rc = GP_E_UNSUPPORTED;
+#define GP_DISPATCH_SMOOTH(P_, C_)                                                                  \
+  if (c->P == P_ && c->C == C_)                                                                     \
+    rc = c->dtype == GP_F64 ? gpk::run_smooth<P_, C_, double>(c, nsweeps) : gpk::run_smooth<P_, C_, float>(c, nsweeps);
+  GP_PC_LIST(GP_DISPATCH_SMOOTH)
+#undef GP_DISPATCH_SMOOTH
+  if (!rc && nsweeps > 0) c->stepped = 1;
+  if (!rc) GP_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
 int gp_sched_info(gp_ctx *c, long *ntiles, long *n_if, long *n_bulk) {
   if (!c) return GP_E_ARG;
   int cnt[2] = {0, 0};

@@ -39,7 +39,7 @@ def lib():
             ("gp_create", [ct.POINTER(GpParams), i, ct.POINTER(vp)]),
             ("gp_set_state", [vp, vp, vp]), ("gp_get_state", [vp, vp, vp]),
             ("gp_set_state_device", [vp, vp, vp]), ("gp_get_state_device", [vp, vp, vp]),
-            ("gp_step", [vp, l]), ("gp_synchronize", [vp]), ("gp_stats_get", [vp, ct.POINTER(GpStats)]),
+            ("gp_step", [vp, l]), ("gp_synchronize", [vp]), ("gp_smooth", [vp, l]), ("gp_stats_get", [vp, ct.POINTER(GpStats)]),
             ("gp_set_stream", [vp, vp]), ("gp_last_step_ms", [vp, ct.POINTER(d)]),
             ("gp_set_path", [vp, i]), ("gp_set_graphs", [vp, i]),
             ("gp_sched_info", [vp, ct.POINTER(l), ct.POINTER(l), ct.POINTER(l)]),
@@ -205,6 +205,10 @@ class Solver:
     def step(self, n=1):
         _check(lib().gp_step(self._h, int(n)), "gp_step")
 
+    def smooth(self, nsweeps):
+        """nsweeps pre-loop smoothing sweeps (an Input.in's NSMOOTH - 1)."""
+        _check(lib().gp_smooth(self._h, int(nsweeps)), "gp_smooth")
+
     def synchronize(self):
         _check(lib().gp_synchronize(self._h), "gp_synchronize")
 
@@ -332,6 +336,10 @@ class Ensemble:
         _check(lib().gp_ensemble_sched_info(self._h, ct.byref(a), ct.byref(b), ct.byref(c), ct.byref(tx),
                                             ct.byref(ty)), "gp_ensemble_sched_info")
         return a.value, b.value, c.value, tx.value, ty.value
+
+    def smooth(self, nsweeps):
+        """nsweeps pre-loop smoothing sweeps (an Input.in's NSMOOTH - 1)."""
+        _check(lib().gp_smooth(self._h, int(nsweeps)), "gp_smooth")
 
     def synchronize(self):
         _check(lib().gp_ensemble_synchronize(self._h), "gp_ensemble_synchronize")

@@ -148,6 +148,10 @@ int  gp_step(gp_ctx *ctx, long nsteps);
 /* Wait for all work of the context (host-side timing without another runtime). */
 int  gp_synchronize(gp_ctx *ctx);
 
+/* nsweeps pre-loop smoothing sweeps (the NSMOOTH-1 calls of smooth() and the
+   ghost copies, microsim_gp.c:97-100; solverloop/solverloop.h:72-90). */
+int  gp_smooth(gp_ctx *ctx, long nsweeps);
+
 /* Diagnostics of the current state. */
 int  gp_stats_get(gp_ctx *ctx, gp_stats *st);
 

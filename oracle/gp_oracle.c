@@ -595,6 +595,133 @@ static void stage_update(const grid *g, const double *phi, const double *mu, wor
     }
 }
 
+void gpo_matinv(double *a, double *inv, int n);
+
+/* ---------------- pre-loop smoothing sweep (NSMOOTH > 1) -------------------
+ * smooth() of solverloop/solverloop.h:72-90: delta-phi without the driving force
+ * (calculate_divergence_phasefield.h:63-114) with the isotropic gradient terms
+ * (function_A_01.h:73-99,220-262 return the isotropic sums also for anisotropic
+ * runs), the obstacle/well terms without sign tests (functionW_01.h:41-63,
+ * functionW_02.h:35-58) and the unweighted projection (simplex_projection.h:50-90);
+ * then the mu update by diffusion alone, general LU inverse for any solver flag
+ * (calculate_divergence_concentration.h:154-253).  delta-phi of a cell enters only
+ * its own update, so the sweep is one pass over the interior. */
+static double dwdphi_smooth(const gp_params *p, const double *ph, const double *divphi, int a) {
+  int P = p->nphases;
+  double sum = 0.0;
+  for (int b = 0; b < P; b++)
+    if (b != a && fabs(divphi[b]) > 0.0) {
+      if (p->obstacle) sum += p->gamma[a][b] * ph[b];
+      else sum += 2.0 * p->gamma[a][b] * (ph[b] * ph[b] * ph[a]);
+    }
+  sum *= p->obstacle ? 16.0 / (M_PI * M_PI) : 9.0;
+  for (int b = 0; b < P; b++)
+    for (int c = 0; c < P; c++)
+      if (b != a && c != a && b < c && fabs(divphi[b]) > 0.0 && fabs(divphi[c]) > 0.0)
+        sum += p->gamma_abc[a][b][c] * ph[b] * ph[c];
+  return sum;
+}
+
+static void projection_unweighted(int P, const double *ph, const double *divphi, double *dp) {
+  double Deltaphi = 0.0;
+  long count = 0;
+  for (int a = 0; a < P; a++)
+    if ((fabs(divphi[a]) > 0.0) && (ph[a] + dp[a]) > 0.0 && (ph[a] + dp[a]) < 1.0) count++;
+  for (int a = 0; a < P; a++) {
+    if (!(fabs(divphi[a]) > 0.0)) continue;
+    if ((ph[a] + dp[a]) > 1.0) {
+      dp[a] = (1.0 - ph[a]);
+      for (int b = 0; b < P; b++)
+        if ((fabs(divphi[b]) > 0.0) && b != a) dp[b] = -ph[b];
+      break;
+    }
+    if ((ph[a] + dp[a]) < 0.0) {
+      Deltaphi = fabs(dp[a] + ph[a]);
+      dp[a] += Deltaphi;
+      for (int b = 0; b < P; b++)
+        if (((fabs(divphi[b]) > 0.0) && b != a) && (ph[b] + dp[b]) > 0.0 && (ph[b] + dp[b]) < 1.0)
+          dp[b] -= Deltaphi / (count);
+    }
+  }
+}
+
+static void stage_smooth(const grid *g, const double *phi, const double *mu, work *w, double *phi_o, double *mu_o) {
+  const gp_params *p = g->p;
+  int P = g->P, C = g->C;
+  long MX = p->mesh_x, MY = p->mesh_y;
+#pragma omp parallel for schedule(static)
+  for (long x = 3; x <= MX + 2; x++)
+    for (long y = 3; y <= MY + 2; y++) {
+      double ph[PM], m[CM], divphi[PM], lam[PM], dp[PM];
+      load_cell(g, phi, mu, x, y, ph, m);
+      for (int a = 0; a < P; a++) {
+        divphi[a] = (AT(g, w->gX, a, x, y) - AT(g, w->gX, a, x - 1, y)) / p->dx;
+        divphi[a] += (AT(g, w->gY, a, x, y) - AT(g, w->gY, a, x, y - 1)) / p->dy;
+      }
+      double sl = 0.0, active = 0.0;
+      for (int a = 0; a < P; a++)
+        if (fabs(divphi[a]) > 0.0) {
+          double dA = 0.0, dv = 0.0;
+          for (int b = 0; b < P; b++) {
+            if (b == a) continue;
+            double sp = 0.0;
+            sp += (ph[a] * AT(g, w->cX, b, x, y) - ph[b] * AT(g, w->cX, a, x, y)) * AT(g, w->cX, b, x, y);
+            sp += (ph[a] * AT(g, w->cY, b, x, y) - ph[b] * AT(g, w->cY, a, x, y)) * AT(g, w->cY, b, x, y);
+            sp *= p->gamma[a][b];
+            dA += sp;
+            double d1 = (FLUXX(a, b, x, y) * (-AT(g, w->sX, b, x, y)) - (FLUXX(a, b, x - 1, y) * (-AT(g, w->sX, b, x - 1, y)))) / p->dx;
+            d1 += (FLUXY(a, b, x, y) * (-AT(g, w->sY, b, x, y)) - (FLUXY(a, b, x, y - 1) * (-AT(g, w->sY, b, x, y - 1)))) / p->dy;
+            d1 *= p->gamma[a][b];
+            dv += d1;
+          }
+          /* function_A_01 *_smooth return 2*(sum1 + sum2) with sum2 = 0; function_A_00 2*sum */
+          double A = p->anisotropy ? 2.0 * (dA + 0.0) : 2.0 * dA;
+          double D = p->anisotropy ? 2.0 * (dv + 0.0) : 2.0 * dv;
+          lam[a] = p->epsilon * (-A + D) - (1.0 / p->epsilon) * dwdphi_smooth(p, ph, divphi, a);
+          sl += lam[a];
+          active++;
+        }
+      if (active) sl /= active;
+      for (int a = 0; a < P; a++) {
+        if (fabs(divphi[a]) > 0.0) dp[a] = p->dt * (lam[a] - sl) / (f_tau(p, ph) * p->epsilon);
+        else dp[a] = 0.0;
+      }
+      projection_unweighted(P, ph, divphi, dp);
+      double divflux[CM];
+      for (int k = 0; k < C; k++) {
+        divflux[k] = 0.0;
+        for (int l = 0; l < C; l++) {
+          divflux[k] += (AT(g, w->DmX, k * C + l, x, y) * AT(g, w->gmX, l, x, y) - AT(g, w->DmX, k * C + l, x - 1, y) * AT(g, w->gmX, l, x - 1, y)) / p->dx;
+          divflux[k] += (AT(g, w->DmY, k * C + l, x, y) * AT(g, w->gmY, l, x, y) - AT(g, w->DmY, k * C + l, x, y - 1) * AT(g, w->gmY, l, x, y - 1)) / p->dy;
+        }
+      }
+      int bulk = -1;
+      for (int a = 0; a < P; a++)
+        if (ph[a] == 1.0) { bulk = a; break; }
+      if (bulk >= 0) {
+        double cn[CM];
+        for (int k = 0; k < C; k++) cn[k] = f_c_mu(p, m, bulk, k) + p->dt * (divflux[k]);
+        for (int k = 0; k < C; k++) AT(g, mu_o, k, x, y) = f_Mu(p, cn, bulk, k);
+      } else {
+        double dc[CM], a16[16], i16[16];
+        for (int k = 0; k < C; k++)
+          for (int l = 0; l < C; l++) a16[k * C + l] = 0.0;
+        for (int k = 0; k < C; k++) {
+          dc[k] = p->dt * (divflux[k]);
+          for (int l = 0; l < C; l++)
+            for (int a = 0; a < P; a++) a16[k * C + l] += p->cmu[a][k][l] * f_hphi(ph, P, a);
+        }
+        gpo_matinv(a16, i16, C);
+        for (int k = 0; k < C; k++) {
+          double s = 0;
+          for (int l = 0; l < C; l++) s = s + i16[k * C + l] * dc[l];
+          AT(g, mu_o, k, x, y) = s + m[k];
+        }
+      }
+      for (int b = 0; b < P; b++) AT(g, phi_o, b, x, y) = ph[b] + dp[b];
+    }
+}
+
 /* LU inverse with partial pivoting of matrix.h:50-114 (same sequence, pow(x,-1)) */
 void gpo_matinv(double *a, double *inv, int n) {
   double fac[16] = {0}, inv1[16], prod[16], v1[4], v[4];
@@ -735,6 +862,42 @@ int gpo_step(const gp_params *p, double *phi, double *mu, long nsteps, int nthre
     stage_mu_faces(&G, phi, mu, &w);
     stage_fluxes(&G, phi, mu, &w);
     stage_update(&G, phi, mu, &w, phi_o, mu_o);
+    memcpy(phi, phi_o, sizeof(double) * G.n * G.P);
+    memcpy(mu, mu_o, sizeof(double) * G.n * G.C);
+    gpo_apply_bc(p, phi, mu);
+  }
+  free_work(&w);
+  free(phi_o);
+  free(mu_o);
+#ifdef _OPENMP
+  omp_set_num_threads(saved);
+#endif
+  return GP_OK;
+}
+
+int gpo_smooth(const gp_params *p, double *phi, double *mu, long nsweeps, int nthreads) {
+  int rc = check(p);
+  if (rc) return rc;
+#ifdef _OPENMP
+  int saved = omp_get_max_threads();
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  grid G = {p, p->mesh_x + 6, p->mesh_y + 6, 0, p->nphases, p->ncomp};
+  G.n = (size_t)G.rx * G.ry;
+  work w;
+  double *phi_o = malloc(sizeof(double) * G.n * G.P), *mu_o = malloc(sizeof(double) * G.n * G.C);
+  if (alloc_work(&G, &w) || !phi_o || !mu_o) {
+    free_work(&w); free(phi_o); free(mu_o);
+    return GP_E_NOMEM;
+  }
+  for (long t = 0; t < nsweeps; t++) {
+    memcpy(phi_o, phi, sizeof(double) * G.n * G.P);
+    memcpy(mu_o, mu, sizeof(double) * G.n * G.C);
+    stage_gradients(&G, phi, &w);
+    stage_mu_faces(&G, phi, mu, &w);
+    stage_smooth(&G, phi, mu, &w, phi_o, mu_o);
     memcpy(phi, phi_o, sizeof(double) * G.n * G.P);
     memcpy(mu, mu_o, sizeof(double) * G.n * G.C);
     gpo_apply_bc(p, phi, mu);

@@ -30,6 +30,10 @@ int gpo_step(const gp_params *p, double *phi, double *mu, long nsteps, int nthre
    (zero elsewhere), for tests (calculate_divergence_phasefield.h:4-62). */
 int gpo_dphi(const gp_params *p, const double *phi, const double *mu, double *dphi);
 
+/* nsweeps pre-loop smoothing sweeps in place (microsim_gp.c:97-100, smooth()
+   of solverloop/solverloop.h:72-90), ghosts refreshed after each. */
+int gpo_smooth(const gp_params *p, double *phi, double *mu, long nsweeps, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

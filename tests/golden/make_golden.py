@@ -120,6 +120,11 @@ CASES = [
          nseeds=3, r=(5, 9), diffuse=1.5, noise=1e-3, steps=60, save=[1, 60]),
     dict(name="p2c1_dendrite_long", P=2, C=1, mesh=(48, 48), aniso=0.02, init="fill",
          fill=["FILLCYLINDER = {0, 24, 24, 0, 0, 5};"], steps=1000, save=[1, 500, 1000]),
+    # pre-loop smoothing (NSMOOTH - 1 sweeps of smooth() before the first step)
+    dict(name="p2c1_smooth_aniso_fill", P=2, C=1, mesh=(48, 40), aniso=0.02, nsmooth=10, init="fill",
+         fill=["FILLCYLINDER = {0, 22, 20, 0, 0, 7};"], steps=20, save=[1, 20]),
+    dict(name="p3c2_smooth_ternary_seeds", P=3, C=2, mesh=(40, 40), solver="TERNARY", nsmooth=6, W=2,
+         gabc=2.0, init="seeds", nseeds=4, r=(4, 8), diffuse=0.0, noise=1e-3, steps=10, save=[1, 10]),
     dict(name="p4c3_general_cube", P=4, C=3, mesh=(32, 32), solver="GENERAL", A_off=0.2,
          init="fill", fill=["FILLCUBE = {0, 4, 4, 0, 12, 12, 0};", "FILLCYLINDER = {1, 22, 20, 0, 0, 6};",
                             "FILLSPHERE = {2, 10, 24, 0, 5};"], steps=30, save=[1, 30]),
@@ -133,6 +138,8 @@ def build_case(c):
     keys = synth.base_params(P, C, (MX, MY))
     keys["NTIMESTEPS"] = str(c["steps"])
     keys["SAVET"] = "1"
+    if c.get("nsmooth"):
+        keys["NSMOOTH"] = str(c["nsmooth"])
     if c.get("aniso"):
         keys["Function_anisotropy"] = "1"
         keys["dab"] = "{" + ", ".join([str(c["aniso"])] * npair) + "}"
@@ -221,6 +228,8 @@ DRIVER_CASES = [
          fill=["FILLCYLINDER = {0, 20, 18, 0, 0, 8};"]),
     dict(name="driver_p3c1_ascii", P=3, C=1, mesh=(32, 36), fmt="ASCII", steps=12, saveT=6,
          fill=["FILLCYLINDER = {0, 10, 12, 0, 0, 6};", "FILLCUBE = {1, 18, 20, 0, 28, 30, 0};"], gabc=1.0),
+    dict(name="driver_p2c1_smooth", P=2, C=1, mesh=(40, 44), fmt="BINARY", steps=10, saveT=5, nsmooth=8,
+         fill=["FILLCYLINDER = {0, 18, 22, 0, 0, 9};"]),
 ]
 
 
@@ -228,7 +237,7 @@ def run_driver_case(c, ref):
     P, C = c["P"], c["C"]
     keys = synth.base_params(P, C, c["mesh"])
     keys.update({"NTIMESTEPS": str(c["steps"]), "SAVET": str(c["saveT"]), "TRACK_PROGRESS": str(c["saveT"]),
-                 "WRITEFORMAT": c["fmt"]})
+                 "WRITEFORMAT": c["fmt"], "NSMOOTH": str(c.get("nsmooth", 1))})
     if c.get("gabc") is not None:
         keys["Gamma_abc"] = "{" + ",".join([str(c["gabc"])] * (P * (P - 1) * (P - 2) // 6)) + "}"
     infile = synth.infile_text(keys, synth.thermo_lines(P, C) + synth.bc_lines())

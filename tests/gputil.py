@@ -22,11 +22,12 @@ def oracle():
         if not os.path.exists(ORACLE_LIB):
             raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
         L = ct.CDLL(ORACLE_LIB)
-        for n in ("gpo_apply_bc", "gpo_step", "gpo_dphi"):
+        for n in ("gpo_apply_bc", "gpo_step", "gpo_dphi", "gpo_smooth"):
             getattr(L, n).restype = ct.c_int
         L.gpo_apply_bc.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p]
         L.gpo_step.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p, ct.c_long, ct.c_int]
         L.gpo_dphi.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.gpo_smooth.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p, ct.c_long, ct.c_int]
         _orc = L
     return _orc
 
@@ -50,6 +51,14 @@ def oracle_run(p, phi, mu, nsteps, nthreads=0):
     """Advance interior fields nsteps with the oracle; returns interior copies."""
     php, mup = pad(p, phi, mu)
     rc = oracle().gpo_step(ct.byref(p), _p(php), _p(mup), int(nsteps), int(nthreads))
+    assert rc == 0, rc
+    return php[:, 3:-3, 3:-3].copy(), mup[:, 3:-3, 3:-3].copy()
+
+
+def oracle_smooth(p, phi, mu, nsweeps, nthreads=0):
+    """nsweeps pre-loop smoothing sweeps with the oracle; returns interior copies."""
+    php, mup = pad(p, phi, mu)
+    rc = oracle().gpo_smooth(ct.byref(p), _p(php), _p(mup), int(nsweeps), int(nthreads))
     assert rc == 0, rc
     return php[:, 3:-3, 3:-3].copy(), mup[:, 3:-3, 3:-3].copy()
 

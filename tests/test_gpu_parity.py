@@ -24,6 +24,8 @@ def test_gpu_f64_reproduces_reference(name, path):
     z, inp, _ = gputil.load_golden(name)
     s = gpstep.Solver(inp, "f64", path=path)
     s.set_state(z["phi_0"], z["mu_0"])
+    if inp.nsmooth > 1:
+        s.smooth(inp.nsmooth - 1)
     done = 0
     for t in z["steps"]:
         s.step(int(t) - done)

@@ -16,7 +16,8 @@ CASES = gputil.golden_cases()
 def test_golden_matrix_is_complete():
     assert len(CASES) >= 12
     names = " ".join(CASES)
-    for tag in ("aniso", "well", "periodic", "ternary", "dilute", "general", "p7", "p4c3", "mixed_bc", "fill"):
+    for tag in ("aniso", "well", "periodic", "ternary", "dilute", "general", "p7", "p4c3", "mixed_bc", "fill",
+                "smooth"):
         assert tag in names, tag
 
 
@@ -25,6 +26,8 @@ def test_oracle_reproduces_reference(name):
     z, inp, fill = gputil.load_golden(name)
     p = inp.p
     phi, mu = z["phi_0"], z["mu_0"]
+    if inp.nsmooth > 1:
+        phi, mu = gputil.oracle_smooth(p, phi, mu, inp.nsmooth - 1, nthreads=4)
     done = 0
     for t in z["steps"]:
         phi, mu = gputil.oracle_run(p, phi, mu, int(t) - done, nthreads=4)
