@@ -219,8 +219,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default=None, help="c1, c2 (default), c3, c4 (ensemble), c5")
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
-    ap.add_argument("--path", type=int, default=0,
-                    help="0 sparse fused, 1 two-pass, 2 dense fused, 3 sparse two-pass, -1 autotune")
+    ap.add_argument("--path", type=int, default=-1,
+                    help="-1 automatic (default), 0 sparse fused, 1 two-pass, 2 dense fused, 3 sparse two-pass")
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="ensemble: keep the default tile shape")
     ap.add_argument("--cpu-steps", type=int, default=None)
@@ -264,20 +264,24 @@ def main():
     P, C, MX, MY = p.nphases, p.ncomp, p.mesh_x, p.mesh_y
     if a.steps is None:
         a.steps = int(min(nsteps_full, max(1, 2e10 // (MX * MY))))
-    s = gpstep.Solver(inp, a.dtype, path=a.path if a.path >= 0 else 0, graphs=not a.no_graphs)
+    s = gpstep.Solver(inp, a.dtype, path=a.path, graphs=not a.no_graphs)
     s.set_state(phi0, mu0)
-    autotuned = a.path < 0
-    if autotuned:
-        a.path, _ = s.autotune(64)
     s.step(a.warmup)
     s.set_state(phi0, mu0)  # the timed steps start from the initial state
+    if a.path < 0:
+        s.set_path(-1)  # restart the automatic choice with the run
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     s.step(a.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     dev_ms = s.last_step_ms()
-    sched = s.sched_info() if a.path != 1 else None
+    names = ["sparse: fused LDS kernel on interface tiles + bulk kernel", "two-pass", "dense fused",
+             "sparse: two-pass kernels on interface tiles + bulk kernel"]
+    path_last, auto = s.path()
+    path_desc = (f"automatic (one timed block of each of two-pass, sparse fused, sparse two-pass every 2048 steps; "
+                 f"last: {names[path_last]})" if auto else names[path_last])
+    sched = s.sched_info() if s.path()[0] != 1 else None
     st = s.stats(allow_nonfinite=True)
     cells = MX * MY
     value = cells * a.steps / wall
@@ -306,9 +310,8 @@ def main():
         "dtype": a.dtype,
         "data": "synthetic (seeded microstructure)",
         "config": {"workload": desc, "mesh_x": MX, "mesh_y": MY, "phases": P, "components": C + 1,
-                   "path": ["sparse: fused LDS kernel on interface tiles + bulk kernel", "two-pass", "dense fused",
-                            "sparse: two-pass kernels on interface tiles + bulk kernel"][a.path],
-                   "tile": list(s.tile_shape()) if a.path != 1 else None, "autotuned": autotuned,
+                   "path": path_desc,
+                   "tile": list(s.tile_shape()) if path_last != 1 else None,
                    "steps_to_solution": nsteps_full},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -62,18 +62,21 @@ __device__ __forceinline__ void ac_eval(R dab, const R* q, AcEval<R>& e) {
     e.H[0][0] = e.H[0][1] = e.H[1][0] = e.H[1][1] = R(0);
     return;
   }
-  const R q4 = qx * qx * qx * qx + qy * qy * qy * qy;
-  const R i2 = R(1) / q2, i4 = i2 * i2, i6 = i4 * i2, i8 = i4 * i4;
-  const R Q = q4 * i4;
+  // in terms of the unit vector n = q/|q| (no powers of 1/|q| beyond the
+  // second, so binary32 stays finite for small |q|)
+  const R r = sqrt(q2), ir = R(1) / r, i2 = ir * ir;
+  const R nx = qx * ir, ny = qy * ir;
+  const R nx2 = nx * nx, ny2 = ny * ny;
+  const R Q = nx2 * nx2 + ny2 * ny2;
   e.aq = -(R(3) - R(4) * Q);
   e.ac = R(1) + dab * e.aq;
-  e.g1[0] = R(16) * (qx * qx * qx * i4 - qx * q4 * i6);
-  e.g1[1] = R(16) * (qy * qy * qy * i4 - qy * q4 * i6);
+  e.g1[0] = R(16) * (nx2 * nx - nx * Q) * ir;
+  e.g1[1] = R(16) * (ny2 * ny - ny * Q) * ir;
   e.g[0] = dab * e.g1[0];
   e.g[1] = dab * e.g1[1];
-  const R hxx = R(12) * qx * qx * i4 - R(32) * qx * qx * qx * qx * i6 - R(4) * q4 * i6 + R(24) * qx * qx * q4 * i8;
-  const R hyy = R(12) * qy * qy * i4 - R(32) * qy * qy * qy * qy * i6 - R(4) * q4 * i6 + R(24) * qy * qy * q4 * i8;
-  const R hxy = -R(16) * qx * qx * qx * qy * i6 - R(16) * qx * qy * qy * qy * i6 + R(24) * qx * qy * q4 * i8;
+  const R hxx = (R(12) * nx2 - R(32) * nx2 * nx2 - R(4) * Q + R(24) * nx2 * Q) * i2;
+  const R hyy = (R(12) * ny2 - R(32) * ny2 * ny2 - R(4) * Q + R(24) * ny2 * Q) * i2;
+  const R hxy = (-R(16) * nx2 * nx * ny - R(16) * nx * ny2 * ny + R(24) * nx * ny * Q) * i2;
   e.H[0][0] = R(4) * dab * hxx;
   e.H[1][1] = R(4) * dab * hyy;
   e.H[0][1] = e.H[1][0] = R(4) * dab * hxy;
@@ -373,7 +376,7 @@ __device__ __forceinline__ void adj_phase_cell(const DevParams<R>& p, const R (&
         if (useSum) {
           Dphib -= g2[b] * ph[b] / sum_phib;
           phb[b] -= g2[b] * Dphi / sum_phib;
-          sumb += g2[b] * Dphi * ph[b] / (sum_phib * sum_phib);
+          sumb += (g2[b] * Dphi * ph[b] / sum_phib) / sum_phib;
         } else {
           Dphib -= g2[b] / R(count);
         }
@@ -538,8 +541,13 @@ __device__ __forceinline__ R sphi0(R f, R& ds) {
   }
   const R h = R(3) * f * f - R(2) * f * f * f, hp = R(6) * f - R(6) * f * f;
   const R sq = sqrt(q);
-  ds = ((R(1) - h) - f * hp) / sq - f * (R(1) - h) * (R(1) - R(2) * f) / (R(2) * q * sq);
-  return f * (R(1) - h) / sq;
+  const R s = f * (R(1) - h) / sq;
+  // d/df of f (1 - h) / sqrt(f (1 - f)), grouped so that no power of q below
+  // the first is formed (binary32 would underflow near f = 0 or 1)
+  const R t = (R(1) - R(2) * f) / (R(2) * q);
+  ds = ((R(1) - h) - f * hp) / sq - s * t;
+  if (!isfinite(ds)) ds = R(0);
+  return s;
 }
 
 // one antitrapping face: p = lower cell, q = upper cell; returns jat and adds
@@ -562,7 +570,7 @@ __device__ __forceinline__ R face_jat_rev(const DevParams<R>& p, R g0, R gL, R c
   }
   const R N = g0 * gL + t * tl;
   const bool nl = ngl > R(0);
-  const R sp = nl ? N / (ngl * ngp) : N;
+  const R sp = nl ? N / (ngl * ngp) : N;  // the forward's expression
   const R cmd = p.cmu[1][0][0], cms = p.cmu[0][0][0];
   const R dcq = cmd * (muq - p.Bsh[1][0]) - cms * (muq - p.Bsh[0][0]);
   const R dcp = cmd * (mup - p.Bsh[1][0]) - cms * (mup - p.Bsh[0][0]);
@@ -575,10 +583,15 @@ __device__ __forceinline__ R face_jat_rev(const DevParams<R>& p, R g0, R gL, R c
   // at_coef = 1 - D_s / D_l
   const R Dl = p.diff_l, Ds = p.diff_s;
   dlb += (double)(jb * asp * X * g0 / ngp * (Ds / (Dl * Dl)));
-  const R spb = jb * A * (sp >= R(0) ? R(1) : R(-1)) * X * g0 / ngp;
-  const R Xb = jb * A * asp * g0 / ngp;
-  g0b += jb * A * asp * X / ngp;
-  R ngpb = -jb * A * asp * X * g0 / (ngp * ngp);
+  // derivatives through the unit vectors (g0, t)/ngp and (gL, tl)/ngl, which stay
+  // bounded where the gradients are small
+  const R u0 = g0 / ngp, ut = t / ngp;
+  const R spb = jb * A * (sp >= R(0) ? R(1) : R(-1)) * X * u0;
+  const R Xb = jb * A * asp * u0;
+  const R ub = jb * A * asp * X;  // adjoint of u0 = g0 / ngp
+  R tb = R(0), tlb = R(0);
+  g0b += ub * ut * ut / ngp;
+  tb -= ub * u0 * ut / ngp;
   // X
   if (p.obstacle) {
     phqb += Xb * pre * dsq * dcq * dpq;
@@ -588,23 +601,18 @@ __device__ __forceinline__ R face_jat_rev(const DevParams<R>& p, R g0, R gL, R c
   mupb += Xb * pre * s_p * dpp * (cmd - cms);
   dpqb += Xb * pre * s_q * dcq;
   dppb += Xb * pre * s_p * dcp;
-  // sp
-  R Nb, nglb = R(0);
+  // sp = (g0 gL + t tl) / (ngp ngl) = u . v with v = (gL, tl) / ngl
   if (nl) {
-    Nb = spb / (ngl * ngp);
-    nglb -= spb * N / (ngl * ngl * ngp);
-    ngpb -= spb * N / (ngl * ngp * ngp);
+    const R v0 = gL / ngl, vt = tl / ngl;
+    g0b += spb * (v0 - sp * u0) / ngp;
+    tb += spb * (vt - sp * ut) / ngp;
+    gLb += spb * (u0 - sp * v0) / ngl;
+    tlb += spb * (ut - sp * vt) / ngl;
   } else {
-    Nb = spb;
-  }
-  g0b += Nb * gL;
-  gLb += Nb * g0;
-  R tb = Nb * tl, tlb = Nb * t;
-  tb += ngpb * t / ngp;
-  g0b += ngpb * g0 / ngp;
-  if (nl) {
-    tlb += nglb * tl / ngl;
-    gLb += nglb * gL / ngl;
+    g0b += spb * gL;
+    gLb += spb * g0;
+    tb += spb * tl;
+    tlb += spb * t;
   }
   ctp0b += R(0.5) * tb;
   ctq0b += R(0.5) * tb;

@@ -6,6 +6,8 @@
 
 #include "gpstep.h"
 
+#define GP_GRAPH_SLOTS 4
+
 struct gp_ctx {
   gp_params hp;         // host copy of the parameters
   int dtype;            // GP_F64 / GP_F32
@@ -40,9 +42,15 @@ struct gp_ctx {
   long n_if_last, n_bulk_last;
   // step graphs: GRAPH_STEPS steps captured once per starting parity
   hipStream_t own_stream;   // created with default priority when the caller sets none
-  hipGraphExec_t graph[2];
+  hipGraphExec_t graph[2];  // unused (see gcache)
   int graph_path;
   int use_graphs;
+  // captured step blocks per (path, tile shape) and starting parity
+  hipGraphExec_t gcache[GP_GRAPH_SLOTS][2];
+  int gkey[GP_GRAPH_SLOTS];
+  // automatic path choice: re-timed every auto_period blocks
+  int auto_mode;
+  long auto_count, auto_period;
   // grids of the tile set: a context steps itself; an ensemble master (nmember
   // > 0) owns no fields and steps its member contexts as one tile set
   int nmember;

@@ -42,8 +42,11 @@ inline void free_tiles(gp_ctx *c) {
   if (c->dcnt) (void)hipFree(c->dcnt), c->dcnt = nullptr;
   if (c->dinst) (void)hipFree(c->dinst), c->dinst = nullptr;
   if (c->dtinst) (void)hipFree(c->dtinst), c->dtinst = nullptr;
-  for (int i = 0; i < 2; i++)
-    if (c->graph[i]) (void)hipGraphExecDestroy(c->graph[i]), c->graph[i] = nullptr;
+  for (int g = 0; g < GP_GRAPH_SLOTS; g++) {
+    for (int i = 0; i < 2; i++)
+      if (c->gcache[g][i]) (void)hipGraphExecDestroy(c->gcache[g][i]), c->gcache[g][i] = nullptr;
+    c->gkey[g] = -1;
+  }
   c->shape_set = -1;
 }
 

@@ -163,8 +163,9 @@ static void free_ctx(gp_ctx *c) {
   if (c->dcnt) (void)hipFree(c->dcnt);
   if (c->dinst) (void)hipFree(c->dinst);
   if (c->dtinst) (void)hipFree(c->dtinst);
-  for (int i = 0; i < 2; i++)
-    if (c->graph[i]) (void)hipGraphExecDestroy(c->graph[i]);
+  for (int g = 0; g < GP_GRAPH_SLOTS; g++)
+    for (int i = 0; i < 2; i++)
+      if (c->gcache[g][i]) (void)hipGraphExecDestroy(c->gcache[g][i]);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -244,6 +245,8 @@ static int ctx_create(const gp_params *p, int dtype, bool own_stream, gp_ctx **o
   c->graph_path = -1;
   c->shape = 0;
   c->shape_set = -1;
+  for (int g = 0; g < GP_GRAPH_SLOTS; g++) c->gkey[g] = -1;
+  c->auto_period = 64;
   *out = c;
   return GP_OK;
 }
@@ -269,8 +272,10 @@ int gp_set_graphs(gp_ctx *c, int on) {
 }
 
 int gp_set_path(gp_ctx *c, int path) {
-  if (!c || path < 0 || path > 3) return GP_E_ARG;
-  c->path = path;
+  if (!c || path < -1 || path > 3 || (c->nmember && path != 3 && path != -1)) return GP_E_ARG;
+  c->auto_mode = path < 0;
+  c->auto_count = 0;
+  c->path = path < 0 ? (c->nmember ? 3 : 1) : path;
   return GP_OK;
 }
 
@@ -325,33 +330,96 @@ static int dispatch_steps(gp_ctx *c, long nsteps) {
   return rc;
 }
 
+// Graph cache slot of the current (path, tile shape).
+static int graph_slot(gp_ctx *c) {
+  const int key = c->path * 16 + c->shape;
+  for (int g = 0; g < GP_GRAPH_SLOTS; g++)
+    if (c->gkey[g] == key) return g;
+  for (int g = 0; g < GP_GRAPH_SLOTS; g++)
+    if (c->gkey[g] < 0) {
+      c->gkey[g] = key;
+      return g;
+    }
+  for (int i = 0; i < 2; i++)
+    if (c->gcache[0][i]) (void)hipGraphExecDestroy(c->gcache[0][i]), c->gcache[0][i] = nullptr;
+  c->gkey[0] = key;
+  return 0;
+}
+
 // Capture GP_GRAPH_STEPS steps starting from the current buffer parity.  The
 // block length is even, so after it the parities (state buffers, tile flags,
 // counters) are back where they started and the graph can be replayed.
-static int capture_graph(gp_ctx *c) {
-  int slot = c->cur;
-  const int key = c->path * 16 + c->shape;
-  if (c->graph[slot] && c->graph_path == key) return GP_OK;
-  if (c->graph_path != key) {
-    for (int i = 0; i < 2; i++)
-      if (c->graph[i]) {
-        (void)hipGraphExecDestroy(c->graph[i]);
-        c->graph[i] = nullptr;
-      }
-    c->graph_path = key;
-  }
-  int cur0 = c->cur, par0 = c->cnt_par;
+static int capture_graph(gp_ctx *c, int gi) {
+  const int slot = c->cur;
+  if (c->gcache[gi][slot]) return GP_OK;
+  int cur0 = c->cur, par0 = c->cnt_par, fv0 = c->flags_valid;
   hipGraph_t g;
   GP_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = dispatch_steps(c, GP_GRAPH_STEPS);
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   c->cur = cur0;
   c->cnt_par = par0;
+  c->flags_valid = fv0;
   if (rc) return rc;
   if (e != hipSuccess) return GP_E_HIP;
-  e = hipGraphInstantiate(&c->graph[slot], g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&c->gcache[gi][slot], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   return e == hipSuccess ? GP_OK : GP_E_HIP;
+}
+
+// One block of GP_GRAPH_STEPS steps with the current path as a graph replay.
+static int run_block(gp_ctx *c) {
+  int rc = dispatch_steps(c, 0);  // tile set and flags of the current state, outside the capture
+  if (rc) return rc;
+  const int gi = graph_slot(c);
+  if ((rc = capture_graph(c, gi))) return rc;
+  if (hipGraphLaunch(c->gcache[gi][c->cur], c->stream) != hipSuccess) return GP_E_HIP;
+  if (c->path == 1) c->flags_valid = 0;  // the two-pass kernels keep no tile flags
+  return GP_OK;
+}
+
+// Automatic path choice: one timed block of each candidate (real steps of the
+// run), then the fastest is kept until the next re-timing.  The candidates share
+// the default tile set, so no graph is invalidated by the switches.
+static int retune(gp_ctx *c, long *left) {
+  const int cand[3] = {1, 0, 3};
+  const int ncand = c->nmember ? 1 : 3;
+  double best = 1e300;
+  int pick = c->path;
+  hipEvent_t a = c->ev0, b = c->ev1;
+  hipEvent_t t0, t1;
+  if (hipEventCreate(&t0) != hipSuccess) return GP_E_HIP;
+  if (hipEventCreate(&t1) != hipSuccess) {
+    (void)hipEventDestroy(t0);
+    return GP_E_HIP;
+  }
+  (void)a;
+  (void)b;
+  int rc = GP_OK;
+  for (int i = 0; i < ncand && *left >= GP_GRAPH_STEPS; i++) {
+    c->path = c->nmember ? 3 : cand[i];
+    c->shape = 0;
+    if ((rc = dispatch_steps(c, 0))) break;
+    const int gi = graph_slot(c);
+    if ((rc = capture_graph(c, gi))) break;
+    if (hipEventRecord(t0, c->stream) != hipSuccess || hipGraphLaunch(c->gcache[gi][c->cur], c->stream) != hipSuccess ||
+        hipEventRecord(t1, c->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess) {
+      rc = GP_E_HIP;
+      break;
+    }
+    if (c->path == 1) c->flags_valid = 0;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    if (ms < best) {
+      best = ms;
+      pick = c->path;
+    }
+    *left -= GP_GRAPH_STEPS;
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  c->path = pick;
+  return rc;
 }
 
 int gp_step(gp_ctx *c, long nsteps) {
@@ -359,14 +427,17 @@ int gp_step(gp_ctx *c, long nsteps) {
   GP_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   int rc = GP_OK;
   long left = nsteps;
-  // graphs: fused paths, on a capturable stream, for runs of at least one block
-  if (c->use_graphs && c->path != 1 && c->stream && left >= GP_GRAPH_STEPS) {
-    rc = dispatch_steps(c, 0);  // tile set-up and flags of a fresh state, outside the capture
+  // graphs: on a capturable stream, for runs of at least one block
+  if (c->use_graphs && c->stream && left >= GP_GRAPH_STEPS) {
     while (!rc && left >= GP_GRAPH_STEPS) {
-      rc = capture_graph(c);
-      if (rc) break;
-      if (hipGraphLaunch(c->graph[c->cur], c->stream) != hipSuccess) return GP_E_HIP;
-      left -= GP_GRAPH_STEPS;  // parities unchanged after an even block
+      if (c->auto_mode && c->auto_count % c->auto_period == 0 && left >= 3 * GP_GRAPH_STEPS) {
+        rc = retune(c, &left);
+        c->auto_count++;
+        continue;
+      }
+      c->auto_count++;
+      rc = run_block(c);
+      left -= GP_GRAPH_STEPS;
     }
   }
   if (!rc && left > 0) rc = dispatch_steps(c, left);
@@ -381,6 +452,13 @@ int gp_step(gp_ctx *c, long nsteps) {
   float ms = 0;
   GP_HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
+  return GP_OK;
+}
+
+int gp_get_path(gp_ctx *c, int *path, int *automatic) {
+  if (!c) return GP_E_ARG;
+  if (path) *path = c->path;
+  if (automatic) *automatic = c->auto_mode;
   return GP_OK;
 }
 
@@ -414,6 +492,7 @@ static int autotune(gp_ctx *c, long steps, bool paths) {
     cand[n++] = {1, 0};
   }
   double best = 1e300;
+  c->auto_mode = 0;
   Cand pick = {c->path, c->shape};
   for (int i = 0; i < n; i++) {
     c->path = cand[i].path;
@@ -545,6 +624,8 @@ int gp_ensemble_create(const gp_params *params, int n, int dtype, gp_ensemble **
   m->graph_path = -1;
   m->shape = 0;
   m->shape_set = -1;
+  for (int g = 0; g < GP_GRAPH_SLOTS; g++) m->gkey[g] = -1;
+  m->auto_period = 64;
   for (int i = 0; i < n; i++) {
     int rc = ctx_create(&params[i], dtype, false, &m->member[i]);
     if (rc) {

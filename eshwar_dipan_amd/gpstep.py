@@ -58,7 +58,7 @@ def lib():
         pi = ct.POINTER(ct.c_int)
         pl = ct.POINTER(l)
         for name, args in [
-            ("gp_set_tile_shape", [vp, i]), ("gp_tile_shape", [vp, pi, pi]),
+            ("gp_set_tile_shape", [vp, i]), ("gp_tile_shape", [vp, pi, pi]), ("gp_get_path", [vp, pi, pi]),
             ("gp_autotune", [vp, l, pi, pi]),
             ("gp_ensemble_create", [ct.POINTER(GpParams), i, i, ct.POINTER(vp)]),
             ("gp_ensemble_size", [vp, pi, pl]),
@@ -205,6 +205,9 @@ class Solver:
     def step(self, n=1):
         _check(lib().gp_step(self._h, int(n)), "gp_step")
 
+    def set_path(self, path):
+        _check(lib().gp_set_path(self._h, int(path)), "gp_set_path")
+
     def smooth(self, nsweeps):
         """nsweeps pre-loop smoothing sweeps (an Input.in's NSMOOTH - 1)."""
         _check(lib().gp_smooth(self._h, int(nsweeps)), "gp_smooth")
@@ -225,6 +228,12 @@ class Solver:
 
     def set_tile_shape(self, shape):
         _check(lib().gp_set_tile_shape(self._h, int(shape)), "gp_set_tile_shape")
+
+    def path(self):
+        """(path in use, automatic choice on)."""
+        pa, au = ct.c_int(), ct.c_int()
+        _check(lib().gp_get_path(self._h, ct.byref(pa), ct.byref(au)), "gp_get_path")
+        return pa.value, bool(au.value)
 
     def tile_shape(self):
         tx, ty = ct.c_int(), ct.c_int()

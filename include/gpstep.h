@@ -166,13 +166,17 @@ int  gp_last_step_ms(gp_ctx *ctx, double *ms);
    diffusion-only kernel on bulk tiles (default), 1 = two-pass kernels (phase
    kernel + concentration kernel over the whole grid), 2 = fused LDS kernel on
    every tile, 3 = two-pass kernels on interface tiles (cells sorted in LDS into
-   interface and bulk cells) plus the bulk kernel.  All paths give bit-identical
-   binary64 results. */
+   interface and bulk cells) plus the bulk kernel, -1 = automatic: every 64
+   blocks of 32 steps, one block of each of paths 1, 0 and 3 is timed and the
+   fastest runs until the next re-timing.  All paths give bit-identical binary64
+   results. */
 int  gp_set_path(gp_ctx *ctx, int path);
+/* The path in use (after automatic choices) and whether the choice is automatic. */
+int  gp_get_path(gp_ctx *ctx, int *path, int *automatic);
 
 /* Replay gp_step calls of 32 steps or more as captured HIP graphs of 32-step
-   blocks (default on; paths 0 and 2).  Capture needs a non-default stream: the
-   context owns one unless gp_set_stream gave another. */
+   blocks (default on).  Capture needs a non-default stream: the context owns one
+   unless gp_set_stream gave another. */
 int  gp_set_graphs(gp_ctx *ctx, int on);
 
 /* Tiles of the last fused step: total, interface (full kernel) and bulk
