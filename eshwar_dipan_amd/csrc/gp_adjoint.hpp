@@ -371,8 +371,10 @@ __device__ __forceinline__ void adj_phase_cell(const DevParams<R>& p, const R (&
   R Dphib = R(0);
   if (redis) {
     R sumb = R(0);
+    // a term whose adjoint is zero contributes nothing (in binary32 the forward
+    // may divide by count = 0 for a phase that the > 1 branch then overwrites)
     for (int b = 0; b < 2; b++)
-      if (inS2[b]) {
+      if (inS2[b] && g2[b] != R(0)) {
         if (useSum) {
           Dphib -= g2[b] * ph[b] / sum_phib;
           phb[b] -= g2[b] * Dphi / sum_phib;
