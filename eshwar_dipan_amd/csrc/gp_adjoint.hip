@@ -47,7 +47,7 @@ __device__ __forceinline__ void load_v(const DevParams<R>& p, const R* phi, int 
 
 // reverse of the mu/phi update, one thread per interior cell
 template <class R>
-__global__ void __launch_bounds__(256) k_adj_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+__global__ void __launch_bounds__(256) GP_HEAVY k_adj_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                   const R* __restrict__ mu, const R* __restrict__ dphi,
                                                   const R* __restrict__ lam_phi, const R* __restrict__ lam_mu,
                                                   R* __restrict__ cg, double* __restrict__ part) {
@@ -88,7 +88,7 @@ __device__ __forceinline__ bool in_interior(int x, int y, int mx, int my) { retu
 // adjoint of delta-phi gathered from the update stencils, then the reverse of
 // the phase sweep, one thread per cell of [2, rows-3]^2
 template <class R>
-__global__ void __launch_bounds__(256) k_adj_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+__global__ void __launch_bounds__(256) GP_HEAVY k_adj_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                    const R* __restrict__ mu, const R* __restrict__ cg,
                                                    R* __restrict__ cf, double* __restrict__ part) {
   __shared__ double red[ADJ_NPAR * 256];
@@ -218,13 +218,32 @@ struct gp_adjoint {
   hipEvent_t e0, e1;
   double fwd_ms, rev_ms;
   long fwd_steps;
+  // device timing without per-step synchronisation: event pairs read at the end
+  std::vector<hipEvent_t> ev;
+  size_t nev;
+  std::vector<int> ev_kind;  // per pair: 0 forward, 1 reverse
+  long cur_step;             // step index of the context's current state (-1: unknown)
+  std::vector<long> slot_step;
   // run state
   bool seeded;
   const double *target;
   double loss;
 };
 
+static int ev_pair(gp_adjoint *A, int kind, hipEvent_t **pair) {
+  while (A->ev.size() < A->nev + 2) {
+    hipEvent_t e;
+    GP_HIP_CHECK(hipEventCreate(&e));
+    A->ev.push_back(e);
+  }
+  *pair = &A->ev[A->nev];
+  A->nev += 2;
+  A->ev_kind.push_back(kind);
+  return GP_OK;
+}
+
 static int adj_save(gp_adjoint *A, int slot) {
+  A->slot_step[slot] = A->cur_step;
   gp_ctx *c = A->c;
   const size_t pb = (size_t)c->plane * c->esize;
   char *d = (char *)A->slots[slot];
@@ -237,6 +256,8 @@ static int adj_save(gp_adjoint *A, int slot) {
 
 static int adj_restore(gp_adjoint *A, int slot) {
   gp_ctx *c = A->c;
+  if (A->cur_step == A->slot_step[slot]) return GP_OK;  // the context holds that state already
+  A->cur_step = A->slot_step[slot];
   const size_t pb = (size_t)c->plane * c->esize;
   const char *d = (const char *)A->slots[slot];
   GP_HIP_CHECK(hipMemcpyAsync(c->phi[c->cur], d, pb * c->P, hipMemcpyDeviceToDevice, c->stream));
@@ -250,9 +271,14 @@ static int adj_restore(gp_adjoint *A, int slot) {
 
 static int adj_advance(gp_adjoint *A, long n) {
   if (n <= 0) return GP_OK;
-  int rc = gp_step(A->c, n);
-  A->fwd_ms += A->c->last_ms;
+  hipEvent_t *e;
+  int rc = ev_pair(A, 0, &e);
+  if (rc) return rc;
+  GP_HIP_CHECK(hipEventRecord(e[0], A->c->stream));
+  rc = gp_step_impl(A->c, n, false);
+  GP_HIP_CHECK(hipEventRecord(e[1], A->c->stream));
   A->fwd_steps += n;
+  A->cur_step += n;
   return rc;
 }
 
@@ -288,7 +314,10 @@ static int adj_step(gp_adjoint *A) {
   using namespace gpk;
   const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
   const R *phi = (const R *)c->phi[c->cur], *mu = (const R *)c->mu[c->cur];
-  GP_HIP_CHECK(hipEventRecord(A->e0, c->stream));
+  hipEvent_t *ep;
+  int rc = ev_pair(A, 1, &ep);
+  if (rc) return rc;
+  GP_HIP_CHECK(hipEventRecord(ep[0], c->stream));
   dim3 blk(BX, BY);
   dim3 g_phase((c->ry - 4 + BX - 1) / BX, (c->rx - 4 + BY - 1) / BY);
   hipLaunchKernelGGL((k_phase<2, 1, R>), g_phase, blk, 0, c->stream, prm, phi, mu, (R *)c->dphi);
@@ -310,11 +339,7 @@ static int adj_step(gp_adjoint *A) {
   hipLaunchKernelGGL((k_adj_fold<R>), g_in, ab, 0, c->stream, prm, (const R *)A->lt, (R *)A->lam[lo][0],
                      (R *)A->lam[lo][1]);
   GP_HIP_CHECK(hipGetLastError());
-  GP_HIP_CHECK(hipEventRecord(A->e1, c->stream));
-  GP_HIP_CHECK(hipEventSynchronize(A->e1));
-  float ms = 0;
-  GP_HIP_CHECK(hipEventElapsedTime(&ms, A->e0, A->e1));
-  A->rev_ms += ms;
+  GP_HIP_CHECK(hipEventRecord(ep[1], c->stream));
   A->lcur = lo;
   return GP_OK;
 }
@@ -426,6 +451,7 @@ void gp_adjoint_destroy(gp_adjoint *A) {
   if (A->acc) (void)hipFree(A->acc);
   if (A->e0) (void)hipEventDestroy(A->e0);
   if (A->e1) (void)hipEventDestroy(A->e1);
+  for (hipEvent_t e : A->ev) (void)hipEventDestroy(e);
   delete A;
 }
 
@@ -438,6 +464,10 @@ int gp_adjoint_run(gp_adjoint *A, const double *phi0, const double *mu0, const d
   A->fwd_ms = A->rev_ms = 0;
   A->fwd_steps = 0;
   A->seeded = false;
+  A->nev = 0;
+  A->ev_kind.clear();
+  A->cur_step = 0;
+  A->slot_step.assign(A->nslots, -1);
   A->target = phi_target;
   GP_HIP_CHECK(hipMemset(A->acc, 0, sizeof(double) * gpk::ADJ_NPAR));
   if ((rc = adj_save(A, 0))) return rc;
@@ -446,6 +476,11 @@ int gp_adjoint_run(gp_adjoint *A, const double *phi0, const double *mu0, const d
   rc = adj_reverse(A, 0, A->nsteps, 0, pool);
   if (rc) return rc;
   GP_HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < A->ev_kind.size(); i++) {
+    float ms = 0;
+    GP_HIP_CHECK(hipEventElapsedTime(&ms, A->ev[2 * i], A->ev[2 * i + 1]));
+    (A->ev_kind[i] ? A->rev_ms : A->fwd_ms) += ms;
+  }
   if (loss) *loss = A->loss;
   if (grad) GP_HIP_CHECK(hipMemcpy(grad, A->acc, sizeof(double) * gpk::ADJ_NPAR, hipMemcpyDeviceToHost));
   if (lam_phi0 || lam_mu0) {

@@ -80,6 +80,9 @@ int run_smooth(gp_ctx *c, long nsweeps);
 
 }  // namespace gpk
 
+// gp_step without the closing synchronisation when timed is false
+int gp_step_impl(gp_ctx *c, long nsteps, bool timed);
+
 #define GP_HIP_CHECK(expr)                 \
   do {                                     \
     hipError_t e_ = (expr);                \

@@ -422,9 +422,15 @@ static int retune(gp_ctx *c, long *left) {
   return rc;
 }
 
-int gp_step(gp_ctx *c, long nsteps) {
+int gp_step(gp_ctx *c, long nsteps) { return gp_step_impl(c, nsteps, true); }
+
+}  // extern "C"
+
+// Steps without the closing synchronisation when `timed` is false (callers that
+// queue more work, such as the adjoint's recomputation).
+int gp_step_impl(gp_ctx *c, long nsteps, bool timed) {
   if (!c || nsteps < 0) return GP_E_ARG;
-  GP_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+  if (timed) GP_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   int rc = GP_OK;
   long left = nsteps;
   // graphs: on a capturable stream, for runs of at least one block
@@ -447,6 +453,7 @@ int gp_step(gp_ctx *c, long nsteps) {
     c->member[i]->cur = c->cur;
     c->member[i]->stepped = c->stepped;
   }
+  if (!timed) return GP_OK;
   GP_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
   GP_HIP_CHECK(hipEventSynchronize(c->ev1));
   float ms = 0;
@@ -454,6 +461,8 @@ int gp_step(gp_ctx *c, long nsteps) {
   c->last_ms = ms;
   return GP_OK;
 }
+
+extern "C" {
 
 int gp_get_path(gp_ctx *c, int *path, int *automatic) {
   if (!c) return GP_E_ARG;
