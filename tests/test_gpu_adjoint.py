@@ -142,3 +142,23 @@ def test_adjoint_binary32_close_to_binary64():
         rel = abs(g["f32"][k] - g["f64"][k]) / max(abs(g["f64"][k]), 1e-30)
         print(f"  d/d{k}: f32 {g['f32'][k]:.6e} f64 {g['f64'][k]:.6e} rel {rel:.2e}", flush=True)
         assert rel < 1e-2, k
+
+
+def test_autograd_function_matches_adjoint_and_gradcheck():
+    """torch.autograd binding: gradients equal the adjoint's, and
+    torch.autograd.gradcheck (central differences) accepts them."""
+    import torch
+    from eshwar_dipan_amd import autograd
+    steps = 12
+    print(f"\n[gpu] torch.autograd binding, 32x28, {steps} steps", flush=True)
+    inp, phi, mu = _case(mesh=(32, 28))
+    p = inp.p
+    target = _target(p, phi, mu, steps)
+    theta = autograd.theta_of(p).requires_grad_(True)
+    loss = autograd.phi_mismatch_loss(theta, phi, mu, target, p, steps)
+    loss.backward()
+    s = gpstep.Solver(p, "f64")
+    _, g, _ = gpstep.Adjoint(s, steps).run(phi, mu, target)
+    assert np.allclose(theta.grad.numpy(), [g[k] for k in gpstep.Adjoint.PARAMS], rtol=1e-14, atol=0)
+    f = lambda th: autograd.phi_mismatch_loss(th, phi, mu, target, p, steps)  # noqa: E731
+    assert torch.autograd.gradcheck(f, (autograd.theta_of(p).requires_grad_(True),), eps=1e-7, atol=1e-6, rtol=1e-4)
