@@ -801,3 +801,150 @@ int gp_vtk_read(const char *path, const gp_input *in, double *phi, double *mu, i
   fclose(f);
   return ok ? GP_OK : GP_E_IO;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Echo files <stem>.out and <stem>.bd                                      */
+/* ------------------------------------------------------------------------ */
+
+/* The key=value formats of the reference's PRINT_* helpers
+   (functions/utility_functions.h:777-837). */
+static void echo_long(FILE *f, const char *k, long v) { fprintf(f, "%s = %ld\n\n", k, v); }
+static void echo_int(FILE *f, const char *k, int v) { fprintf(f, "%s = %d\n\n", k, v); }
+static void echo_double(FILE *f, const char *k, double v) { fprintf(f, "%s = %le\n\n", k, v); }
+
+/* matrix rows continue under the first value: the key plus "= [ " wide */
+static void echo_matrix(FILE *f, const char *k, const double *m, int rows, int cols, int stride) {
+  fprintf(f, "%s = [", k);
+  for (int i = 0; i < rows; i++) {
+    for (int j = 0; j < cols; j++) fprintf(f, j < cols - 1 ? "%le," : "%le", m[i * stride + j]);
+    if (i < rows - 1) fprintf(f, "\n%*s", (int)strlen(k) + 4, "");
+    else fprintf(f, "]\n");
+  }
+  fprintf(f, "\n");
+}
+
+static void echo_vector(FILE *f, const char *k, const double *v, int n) {
+  fprintf(f, "%s = [", k);
+  for (int i = 0; i < n; i++) fprintf(f, i < n - 1 ? "%le," : "%le]\n", v[i]);
+  fprintf(f, "\n");
+}
+
+int gp_write_echo(const char *infile, const gp_input *in) {
+  if (!infile || !in) return GP_E_ARG;
+  /* <stem> = the part of the Input.in path before its first '.' (reading_input_parameters.h:310-316) */
+  char stem[1024], name[1100], key[256];
+  snprintf(stem, sizeof stem, "%s", infile);
+  char *dot = strchr(stem, '.');
+  if (dot) *dot = '\0';
+  const gp_params *p = &in->p;
+  const int P = p->nphases, C = p->ncomp, L = P - 1;
+  snprintf(name, sizeof name, "%s.out", stem);
+  FILE *f = fopen(name, "w");
+  if (!f) return GP_E_IO;
+  echo_long(f, "MESH_X", p->mesh_x);
+  echo_long(f, "MESH_Y", p->mesh_y);
+  echo_long(f, "MESH_Z", in->mesh_z);
+  echo_double(f, "DELTA_X", p->dx);
+  echo_double(f, "DELTA_Y", p->dy);
+  echo_double(f, "DELTA_Z", in->dz);
+  echo_int(f, "NUMPHASES", P);
+  echo_int(f, "NUMCOMPONENTS", C + 1);
+  echo_int(f, "DIMENSION", in->dimension);
+  echo_long(f, "NTIMESTEPS", in->ntimesteps);
+  echo_long(f, "SAVET", in->saveT);
+  echo_long(f, "NSMOOTH", in->nsmooth);
+  echo_long(f, "STARTTIME", in->starttime);
+  echo_long(f, "RESTART", in->restart);
+  echo_double(f, "R", in->R);
+  echo_double(f, "V", in->V);
+  echo_double(f, "TEMPERATURE_SCALE", in->temperature_scale);
+  echo_double(f, "TEMPERATURE", p->T);
+  echo_double(f, "Equilibrium_temperature", p->Teq);
+  echo_double(f, "Filling_temperature", in->Tfill);
+  echo_matrix(f, "GAMMA", &p->gamma[0][0], P, P, GP_MAX_PHASES);
+  echo_matrix(f, "dab", &p->dab[0][0], P, P, GP_MAX_PHASES);
+  echo_matrix(f, "Tau", &p->tau_ab[0][0], P, P, GP_MAX_PHASES);
+  for (int i = 0; i < P; i++) {
+    snprintf(key, sizeof key, "DIFFUSIVITY[%s]", in->phases[i]);
+    echo_matrix(f, key, &p->diffusivity[i][0][0], C, C, GP_MAX_COMP);
+  }
+  for (int i = 0; i < P; i++) {
+    snprintf(key, sizeof key, "Slopes[Solidus,%s]", in->phases[i]);
+    echo_vector(f, key, p->slopes[i][i], C);
+  }
+  for (int i = 0; i < P; i++) {
+    snprintf(key, sizeof key, "Slopes[Liquidus,%s]", in->phases[i]);
+    echo_vector(f, key, p->slopes[i][L], C);
+  }
+  for (int i = 0; i < P; i++) {
+    snprintf(key, sizeof key, "A[%s]", in->phases[i]);
+    echo_matrix(f, key, &p->A[i][0][0], C, C, GP_MAX_COMP);
+  }
+  for (int i = 0; i < P; i++)
+    for (int j = 0; j < P; j++)
+      if (i != j) {
+        snprintf(key, sizeof key, "Rotation_matrix[%s][%s]", in->phases[i], in->phases[j]);
+        echo_matrix(f, key, &p->rot[i][j][0][0], 3, 3, 3);
+      }
+  for (int i = 0; i < P; i++)
+    for (int j = 0; j < P; j++)
+      if (i != j) {
+        snprintf(key, sizeof key, "Inv_Rotation_matrix[%s][%s]", in->phases[i], in->phases[j]);
+        echo_matrix(f, key, &p->rot_inv[i][j][0][0], 3, 3, 3);
+      }
+  echo_int(f, "Function_F", in->function_f);
+  echo_int(f, "Function_W", in->function_w);
+  echo_int(f, "Function_anisotropy", in->function_anisotropy);
+  echo_int(f, "Anisotropy_type", in->fold);
+  fprintf(f, "COMPONENTS = [");
+  for (int k = 0; k <= C; k++) fprintf(f, k < C ? "%s," : "%s]\n", in->components[k]);
+  fprintf(f, "\n");
+  /* the reference prints NUMCOMPONENTS phase names here (reading_input_parameters.h:474) */
+  fprintf(f, "PHASES = [");
+  for (int k = 0; k <= C; k++) fprintf(f, k < C ? "%s," : "%s]\n", k < P ? in->phases[k] : "");
+  fprintf(f, "\n");
+  for (int i = 0; i < P; i++) {
+    snprintf(key, sizeof key, "ceq[Solidus,%s]", in->phases[i]);
+    echo_vector(f, key, p->ceq[i][i], C);
+  }
+  for (int i = 0; i < P; i++) {
+    snprintf(key, sizeof key, "ceq[Liquidus,%s]", in->phases[i]);
+    echo_vector(f, key, p->ceq[i][L], C);
+  }
+  for (int i = 0; i < P; i++) {
+    snprintf(key, sizeof key, "cfill[Liquidus,%s]", in->phases[i]);
+    echo_vector(f, key, p->cfill[i][L], C);
+  }
+  /* elasticity is not part of the step: eigen strains and stiffnesses print as zero */
+  static const char *comp6[6] = {"xx", "yy", "zz", "yz", "xz", "xy"};
+  for (int i = 0; i < P; i++) {
+    for (int c = 0; c < 6; c++) fprintf(f, "EIGEN_STRAIN[%s].%s = %le\n", in->phases[i], comp6[c], 0.0);
+    fprintf(f, "\n");
+  }
+  static const char *stiff[3] = {"C11", "C12", "C44"};
+  for (int i = 0; i < P; i++) {
+    for (int c = 0; c < 3; c++) fprintf(f, "Stiffness_cubic[%s].%s = %le\n", in->phases[i], stiff[c], 0.0);
+    fprintf(f, "\n");
+  }
+  fclose(f);
+
+  /* boundary echo (functions/utility_functions.h:875-916, read_boundary_conditions.h:50-55) */
+  snprintf(name, sizeof name, "%s.bd", stem);
+  f = fopen(name, "w");
+  if (!f) return GP_E_IO;
+  static const char *scal[3] = {"PHI", "MU", "T"};
+  static const char *face[6] = {"LEFT", "RIGHT", "FRONT", "BACK", "TOP", "BOTTOM"};
+  static const char *tname[4] = {"FREE", "NEUMANN", "DIRICHLET", "PERIODIC"};
+  const int order[4] = {GP_FACE_XMIN, GP_FACE_XMAX, GP_FACE_YMAX, GP_FACE_YMIN};
+  for (int j = 0; j < 3; j++)
+    for (int i = 0; i < 6; i++) {
+      int ty = 0;
+      if (i < 4) ty = j < 2 ? p->bc_type[j][order[i]] : in->bc_type_T[order[i]];
+      if (ty < 0 || ty > 3) continue;
+      fprintf(f, "BOUNDARY_%s[%s] = %s\n\n", face[i], scal[j], tname[ty]);
+      if (ty == 0 || ty == 2)
+        fprintf(f, "DIRICHLET and FREE boundary conditions have not yet been implemented. Please wait for next release. Will default to PERIODIC\n");
+    }
+  fclose(f);
+  return GP_OK;
+}

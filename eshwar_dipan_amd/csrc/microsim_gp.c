@@ -45,6 +45,8 @@ int main(int argc, char *argv[]) {
   }
   rc = gp_derive(&in.p);
   if (rc) return fail("deriving constants", rc);
+  rc = gp_write_echo(argv[1], &in);
+  if (rc) return fail("writing the parameter echo", rc);
   rc = gp_check_supported(&in.p, err, sizeof err);
   if (rc) {
     fprintf(stderr, "microsim_gp: %s\n", err);

@@ -86,6 +86,12 @@ void gp_matinv(double *a, double *inv, int n);
    library, so that bindings can check their structure layouts. */
 int gp_abi_sizes(size_t *params, size_t *stats, size_t *input);
 
+/* Write the parameter echo <stem>.out and the boundary echo <stem>.bd, <stem>
+   being the Input.in path up to its first '.' (reading_input_parameters.h:310-500,
+   read_boundary_conditions.h:46-55).  Eigen strains and stiffnesses, which the
+   step does not use, print as zero. */
+int gp_write_echo(const char *infile, const gp_input *in);
+
 /* Chemical potential Mu_a(c) at temperature T (functionF_01.h:19-29). */
 double gp_mu_of_c(const gp_params *p, const double *c, double T, int a, int i);
 

@@ -3,7 +3,7 @@ output files.
 
 Each fixture holds the Input.in / Filling.in of a short run and the files the
 reference serial solver wrote for it (DATA/out_<t>.vtk in BINARY or ASCII,
-DATA/shift.dat) plus its stdout.  The driver is run on the same two input files
+DATA/shift.dat, the Input.out / Input.bd echo files) plus its stdout.  The driver is run on the same two input files
 in an empty directory; every DATA file must be byte-identical and the progress
 report must print the same times, extrema and (to the documented tolerance)
 relative changes.
@@ -54,6 +54,9 @@ def test_driver_reproduces_reference_files(case):
             got = np.frombuffer(open(os.path.join(d, "DATA", f), "rb").read(), dtype=np.uint8)
             want = z["file_" + f]
             assert got.shape == want.shape and np.array_equal(got, want), f"{case}: DATA/{f} differs"
+        for ext in ("out", "bd"):
+            got = open(os.path.join(d, f"Input.{ext}"), "rb").read()
+            assert got == bytes(z[f"echo_Input_{ext}"]), f"{case}: Input.{ext} differs"
         ours, ref = _report(r.stdout.decode()), _report(bytes(z["stdout"]).decode())
         assert len(ours) == len(ref) and len(ref) > 0
         # The reference folds extrema into running values at every step and

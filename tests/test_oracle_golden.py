@@ -5,6 +5,8 @@ its sources (tests/golden/make_golden.py); the oracle, fed the same Input.in
 through the product's parser and constant derivation, must reproduce every saved
 state bit for bit.  Also pins the product's Filling.in path against the
 reference's t = 0 output."""
+import os
+
 import numpy as np
 import pytest
 
@@ -55,3 +57,22 @@ def test_oracle_detects_a_planted_error():
     p.dab[1][0] = p.dab[0][1]
     phi, mu = gputil.oracle_run(p, z["phi_0"], z["mu_0"], 1, nthreads=2)
     assert not np.array_equal(phi, z["phi_1"])
+
+
+@pytest.mark.parametrize("case", gputil.driver_cases())
+def test_echo_files_match_reference(case, tmp_path):
+    """Input.out / Input.bd written by gp_write_echo equal the reference's bytes."""
+    import ctypes as ct
+    from eshwar_dipan_amd import gpstep
+    z = np.load(os.path.join(gputil.GOLDEN, case + ".npz"))
+    src = tmp_path / "Input.in"
+    src.write_text(str(z["infile"]))
+    inp = gpstep.read_input(str(src))
+    L = gpstep.lib()
+    L.gp_write_echo.argtypes = [ct.c_char_p, ct.c_void_p]
+    L.gp_write_echo.restype = ct.c_int
+    assert L.gp_write_echo(str(src).encode(), ct.byref(inp)) == 0
+    for ext, key in (("out", "echo_Input_out"), ("bd", "echo_Input_bd")):
+        got = (tmp_path / f"Input.{ext}").read_bytes()
+        want = bytes(z[key])
+        assert got == want, f"{case}: Input.{ext} differs at byte {next((i for i, (a, b) in enumerate(zip(got, want)) if a != b), min(len(got), len(want)))}"
