@@ -121,15 +121,12 @@ __global__ void __launch_bounds__(256) GP_HEAVY k_adj_phase(const DevParams<R>* 
     for (int k = 0; k < ADJ_NPAR; k++) part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * ADJ_NPAR + k] = par[k];
 }
 
-// sum of the stencil partials pointing at cell j, for j in [1, rows-2]^2
+// Sum of the stencil partials pointing at cell (x, y) of [1, rows-2]^2:
+// sp[0..1] for phi, sm for mu.
 template <class R>
-__global__ void __launch_bounds__(256) k_adj_gather(const DevParams<R>* __restrict__ prm, const R* __restrict__ cg,
-                                                    const R* __restrict__ cf, R* __restrict__ lt) {
-  const DevParams<R>& p = *prm;
-  const int y = 1 + blockIdx.x * ABX + threadIdx.x;
-  const int x = 1 + blockIdx.y * ABY + threadIdx.y;
-  if (x > p.rx - 2 || y > p.ry - 2) return;
-  R sp[2] = {R(0), R(0)}, sm = R(0);
+__device__ __forceinline__ void gather_at(const DevParams<R>& p, const R* __restrict__ cg, const R* __restrict__ cf,
+                                          int x, int y, R* sp, R& sm) {
+  sp[0] = sp[1] = sm = R(0);
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) {
       const int cx = x - (i - 1), cy = y - (j - 1);
@@ -144,22 +141,23 @@ __global__ void __launch_bounds__(256) k_adj_gather(const DevParams<R>* __restri
     const int cx = x - D5X[d], cy = y - D5Y[d];
     if (in_interior(cx, cy, p.mesh_x, p.mesh_y)) sm += cg[(size_t)(CG_MU + d) * p.plane + (size_t)cx * p.pitch + cy];
   }
-  const size_t o = (size_t)x * p.pitch + y;
-  if (x >= 2 && x <= p.rx - 3 && y >= 2 && y <= p.ry - 3) sm += cf[(size_t)CF_MU * p.plane + o];
-  lt[o] = sp[0];
-  lt[p.plane + o] = sp[1];
-  lt[2 * p.plane + o] = sm;
+  if (x >= 2 && x <= p.rx - 3 && y >= 2 && y <= p.ry - 3) sm += cf[(size_t)CF_MU * p.plane + (size_t)x * p.pitch + y];
 }
 
-// reverse of the ghost copies: an interior cell collects the adjoints of its
-// images (the cells write_with_ghosts writes from it)
+// New adjoint of every interior cell: the partials pointing at it plus those
+// pointing at its ghost images (the reverse of the ghost copies, which write
+// an interior value to the cells write_with_ghosts names).
 template <class R>
-__global__ void __launch_bounds__(256) k_adj_fold(const DevParams<R>* __restrict__ prm, const R* __restrict__ lt,
-                                                  R* __restrict__ lam_phi, R* __restrict__ lam_mu) {
+__global__ void __launch_bounds__(256) k_adj_gather(const DevParams<R>* __restrict__ prm, const R* __restrict__ cg,
+                                                    const R* __restrict__ cf, R* __restrict__ lam_phi,
+                                                    R* __restrict__ lam_mu) {
   const DevParams<R>& p = *prm;
   const int y = 3 + blockIdx.x * ABX + threadIdx.x;
   const int x = 3 + blockIdx.y * ABY + threadIdx.y;
   if (x > p.mesh_x + 2 || y > p.mesh_y + 2) return;
+  R sp[2], sm;
+  gather_at(p, cg, cf, x, y, sp, sm);
+  // images: ghost rows/columns whose copy source is x / y, per field
   for (int f = 0; f < 2; f++) {
     int tx[3], ty[3], nx = 1, ny = 1;
     tx[0] = x;
@@ -168,20 +166,25 @@ __global__ void __launch_bounds__(256) k_adj_fold(const DevParams<R>* __restrict
       if (p.gsrc[f][0][s] == x) tx[nx++] = ghost_index(s, p.rx);
       if (p.gsrc[f][1][s] == y) ty[ny++] = ghost_index(s, p.ry);
     }
-    const int nfield = f == 0 ? 2 : 1;
-    for (int k = 0; k < nfield; k++) {
-      const int plane = f == 0 ? k : 2;
-      R s = R(0);
-      for (int i = 0; i < nx; i++)
-        for (int j = 0; j < ny; j++) {
-          const int gx = tx[i], gy = ty[j];
-          if (gx < 1 || gx > p.rx - 2 || gy < 1 || gy > p.ry - 2) continue;  // never read by a stencil
-          s += lt[(size_t)plane * p.plane + (size_t)gx * p.pitch + gy];
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++) {
+        if (i == 0 && j == 0) continue;
+        const int gx = tx[i], gy = ty[j];
+        if (gx < 1 || gx > p.rx - 2 || gy < 1 || gy > p.ry - 2) continue;  // never read by a stencil
+        R gp[2], gm;
+        gather_at(p, cg, cf, gx, gy, gp, gm);
+        if (f == 0) {
+          sp[0] += gp[0];
+          sp[1] += gp[1];
+        } else {
+          sm += gm;
         }
-      if (f == 0) lam_phi[(size_t)k * p.plane + (size_t)x * p.pitch + y] = s;
-      else lam_mu[(size_t)x * p.pitch + y] = s;
-    }
+      }
   }
+  const size_t o = (size_t)x * p.pitch + y;
+  lam_phi[o] = sp[0];
+  lam_phi[p.plane + o] = sp[1];
+  lam_mu[o] = sm;
 }
 
 // fixed-order sum of the block partials into the gradient accumulator
@@ -210,7 +213,7 @@ struct gp_adjoint {
   int nslots;
   size_t slot_bytes;     // bytes of one saved state (phi, mu and mu low words, padded planes)
   std::vector<void *> slots;
-  void *cg, *cf, *lt;    // partial planes, gathered adjoints
+  void *cg, *cf;         // partial planes of the update and phase reverses
   void *lam[2][2];       // [buffer][0 = phi (2 planes), 1 = mu]
   int lcur;
   double *part, *acc;
@@ -307,13 +310,14 @@ static int adj_seed(gp_adjoint *A) {
   return GP_OK;
 }
 
-// one reverse step from the context's current state S_k: lam <- J_k^T lam
+// one reverse step from the state S_k at (phi, mu) (the context's buffers or
+// a snapshot slot, same padded layout): lam <- J_k^T lam
 template <class R>
-static int adj_step(gp_adjoint *A) {
+static int adj_step(gp_adjoint *A, const void *phi_s, const void *mu_s) {
   gp_ctx *c = A->c;
   using namespace gpk;
   const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
-  const R *phi = (const R *)c->phi[c->cur], *mu = (const R *)c->mu[c->cur];
+  const R *phi = (const R *)phi_s, *mu = (const R *)mu_s;
   hipEvent_t *ep;
   int rc = ev_pair(A, 1, &ep);
   if (rc) return rc;
@@ -325,27 +329,33 @@ static int adj_step(gp_adjoint *A) {
   dim3 ab(ABX, ABY);
   dim3 g_in((MY + ABX - 1) / ABX, (MX + ABY - 1) / ABY);
   dim3 g_ring((c->ry - 4 + ABX - 1) / ABX, (c->rx - 4 + ABY - 1) / ABY);
-  dim3 g_all((c->ry - 2 + ABX - 1) / ABX, (c->rx - 2 + ABY - 1) / ABY);
   const int li = A->lcur, lo = A->lcur ^ 1;
   const int nb_in = (int)(g_in.x * g_in.y), nb_ring = (int)(g_ring.x * g_ring.y);
   hipLaunchKernelGGL((k_adj_conc<R>), g_in, ab, 0, c->stream, prm, phi, mu, (const R *)c->dphi,
                      (const R *)A->lam[li][0], (const R *)A->lam[li][1], (R *)A->cg, A->part);
-  hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(256), 0, c->stream, (const double *)A->part, nb_in, A->acc);
   hipLaunchKernelGGL((k_adj_phase<R>), g_ring, ab, 0, c->stream, prm, phi, mu, (const R *)A->cg, (R *)A->cf,
-                     A->part);
-  hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(256), 0, c->stream, (const double *)A->part, nb_ring, A->acc);
-  hipLaunchKernelGGL((k_adj_gather<R>), g_all, ab, 0, c->stream, prm, (const R *)A->cg, (const R *)A->cf,
-                     (R *)A->lt);
-  hipLaunchKernelGGL((k_adj_fold<R>), g_in, ab, 0, c->stream, prm, (const R *)A->lt, (R *)A->lam[lo][0],
-                     (R *)A->lam[lo][1]);
+                     A->part + (size_t)nb_in * ADJ_NPAR);
+  hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(256), 0, c->stream, (const double *)A->part, nb_in + nb_ring, A->acc);
+  hipLaunchKernelGGL((k_adj_gather<R>), g_in, ab, 0, c->stream, prm, (const R *)A->cg, (const R *)A->cf,
+                     (R *)A->lam[lo][0], (R *)A->lam[lo][1]);
   GP_HIP_CHECK(hipGetLastError());
   GP_HIP_CHECK(hipEventRecord(ep[1], c->stream));
   A->lcur = lo;
   return GP_OK;
 }
 
-static int adj_step_any(gp_adjoint *A) {
-  return A->c->dtype == GP_F64 ? adj_step<double>(A) : adj_step<float>(A);
+// reverse of step k: from the slot when it holds S_k, else from the context
+static int adj_step_any(gp_adjoint *A, int sl, long k) {
+  gp_ctx *c = A->c;
+  const void *phi = c->phi[c->cur], *mu = c->mu[c->cur];
+  if (sl >= 0 && A->slot_step[sl] == k) {
+    const size_t pb = (size_t)c->plane * c->esize;
+    phi = A->slots[sl];
+    mu = (const char *)A->slots[sl] + pb * c->P;
+  } else if (A->cur_step != k) {
+    return GP_E_ARG;
+  }
+  return c->dtype == GP_F64 ? adj_step<double>(A, phi, mu) : adj_step<float>(A, phi, mu);
 }
 
 static long binom(int s, int t) {
@@ -362,16 +372,19 @@ static long binom(int s, int t) {
 // adjoint from S_{k+1} first if this is the last step, then reverse.
 static int adj_one(gp_adjoint *A, long lo, long k, int sl) {
   int rc;
-  if ((rc = adj_restore(A, sl))) return rc;
-  if ((rc = adj_advance(A, k - lo))) return rc;
   if (!A->seeded) {
+    if (A->cur_step != k) {
+      if ((rc = adj_restore(A, sl))) return rc;
+      if ((rc = adj_advance(A, k - lo))) return rc;
+    }
     if ((rc = adj_advance(A, 1))) return rc;
     rc = A->c->dtype == GP_F64 ? adj_seed<double>(A) : adj_seed<float>(A);
     if (rc) return rc;
-    if ((rc = adj_restore(A, sl))) return rc;
-    if ((rc = adj_advance(A, k - lo))) return rc;
   }
-  return adj_step_any(A);
+  if (k == lo) return adj_step_any(A, sl, k);  // the slot holds S_k: no copy back
+  if ((rc = adj_restore(A, sl))) return rc;
+  if ((rc = adj_advance(A, k - lo))) return rc;
+  return adj_step_any(A, -1, k);
 }
 
 // Reverse steps [lo, hi) with S_lo in slot `sl` and the free slots in `pool`
@@ -421,16 +434,14 @@ int gp_adjoint_create(gp_ctx *c, long nsteps, int nsnap, gp_adjoint **out) {
   bool ok = true;
   A->slots.assign(A->nslots, nullptr);
   for (int i = 0; i < A->nslots && ok; i++) ok = hipMalloc(&A->slots[i], A->slot_bytes) == hipSuccess;
-  ok = ok && hipMalloc(&A->cg, pb * gpk::CG_N) == hipSuccess && hipMalloc(&A->cf, pb * gpk::CF_N) == hipSuccess &&
-       hipMalloc(&A->lt, pb * 3) == hipSuccess;
+  ok = ok && hipMalloc(&A->cg, pb * gpk::CG_N) == hipSuccess && hipMalloc(&A->cf, pb * gpk::CF_N) == hipSuccess;
   for (int b = 0; b < 2 && ok; b++)
     ok = hipMalloc(&A->lam[b][0], pb * 2) == hipSuccess && hipMalloc(&A->lam[b][1], pb) == hipSuccess &&
          hipMemset(A->lam[b][0], 0, pb * 2) == hipSuccess && hipMemset(A->lam[b][1], 0, pb) == hipSuccess;
   const int gx = (c->ry - 2 + gpk::ABX - 1) / gpk::ABX, gy = (c->rx - 2 + gpk::ABY - 1) / gpk::ABY;
-  A->nblk = gx * gy;
+  A->nblk = 2 * gx * gy;  // partials of the update reverse and of the phase reverse
   ok = ok && hipMalloc((void **)&A->part, sizeof(double) * gpk::ADJ_NPAR * A->nblk) == hipSuccess &&
        hipMalloc((void **)&A->acc, sizeof(double) * gpk::ADJ_NPAR) == hipSuccess;
-  ok = ok && hipMemset(A->lt, 0, pb * 3) == hipSuccess;
   ok = ok && hipEventCreate(&A->e0) == hipSuccess && hipEventCreate(&A->e1) == hipSuccess;
   if (!ok) {
     gp_adjoint_destroy(A);
@@ -445,7 +456,7 @@ void gp_adjoint_destroy(gp_adjoint *A) {
   if (A->c) (void)hipStreamSynchronize(A->c->stream);
   for (void *s : A->slots)
     if (s) (void)hipFree(s);
-  for (void *p : {A->cg, A->cf, A->lt, A->lam[0][0], A->lam[0][1], A->lam[1][0], A->lam[1][1]})
+  for (void *p : {A->cg, A->cf, A->lam[0][0], A->lam[0][1], A->lam[1][0], A->lam[1][1]})
     if (p) (void)hipFree(p);
   if (A->part) (void)hipFree(A->part);
   if (A->acc) (void)hipFree(A->acc);

@@ -5,7 +5,7 @@
 // The step is S_{n+1} = B(G(S_n, F(S_n))): F is the delta-phi of every cell of
 // the interior plus its ghost ring (solverloop_phasefield), G the mu and phi
 // update of the interior cells (solverloop_concentration), B the ghost copies.
-// Its adjoint is applied as four gathers, so no cell is written by two threads:
+// Its adjoint is applied as three gathers, so no cell is written by two threads:
 //
 //   k_adj_conc   per interior cell c: reverse of G_c seeded with the adjoint of
 //                its new (phi, mu); writes the partials of G_c times the seed
@@ -13,9 +13,8 @@
 //   k_adj_phase  per ring/interior cell i: gathers the adjoint of delta-phi(i)
 //                from the (up to) five G-stencils that read it, then reverses
 //                F_i; writes its stencil partials likewise;
-//   k_adj_gather per cell: sums the stencil partials that point at it;
-//   k_adj_fold   per interior cell: adds the adjoints of its ghost images
-//                (the reverse of B).
+//   k_adj_gather per interior cell: sums the stencil partials that point at
+//                it and at its ghost images (the reverse of B).
 //
 // Each cell program recomputes its forward values from S_n and delta-phi(S_n)
 // and differentiates the mathematical expressions of the forward kernels
