@@ -28,6 +28,7 @@ struct DevParams {
   int mesh_x, mesh_y;
   int obstacle, anisotropy, mu_solver;
   int yskip;              // padded y index MESH_Y-2 whose y-face antitrapping is skipped
+  int pair_sym;           // dab, rot and rot_inv are symmetric in the phase pair
   int pow2_dx, pow2_dy;   // spacing is a power of two (division = exact multiplication)
   int gsrc[2][2][6];      // [field][axis][ghost slot] -> padded source index (slots: 0,1,2,n-3,n-2,n-1)
   R dx, dy, dt, two_dy;
@@ -385,6 +386,109 @@ __device__ __forceinline__ R divdAdgradphi(const DevParams<R>& p, const Nbhd<P, 
   return (R(2.0) * (sum1 + sum2) + R(0));
 }
 
+// dA/dphi_a and div(dA/dgrad phi_a) of the anisotropic form for every phase a
+// with want[a], pair by pair.  With dab, rot and rot_inv symmetric in the
+// pair, the ordered pair (b,a) sees the vector u of (a,b) negated: q = R u,
+// da_c/dq and R^-1 da_c/dq change sign, a_c and |q|^2 do not (a_c is even in q,
+// da_c/dq odd, and IEEE negation commutes with the products and sums involved).
+// So the a_c, da_c/dq and face terms of each unordered pair are evaluated once
+// and used with a sign for both phases.  Visiting pairs (a,b), a < b, in
+// ascending order hands every phase its partners in ascending order, the order
+// of dAdphi / divdAdgradphi above, so each per-phase sum is formed from the
+// same terms in the same order.  Terms may differ from the per-phase functions
+// only in the sign of a zero, which leaves nonzero sums unchanged; the results
+// are returned as dA = 2(s1+s2) (entering lam as -dA + dG, where either zero
+// gives +0) and dG = 2(s1+s2) + 0 (zero normalised as in divdAdgradphi).
+template <int P, class R>
+__device__ __forceinline__ void aniso_pairs(const DevParams<R>& p, const Nbhd<P, R>& n, const bool* want, R* dA,
+                                            R* dG) {
+  R a1[P], a2[P], g1[P], g2[P];
+#pragma unroll
+  for (int a = 0; a < P; a++) a1[a] = a2[a] = g1[a] = g2[a] = R(0);
+#pragma unroll
+  for (int a = 0; a < P; a++)
+#pragma unroll
+    for (int b = a + 1; b < P; b++) {
+      const bool ua = want[a] && n.present[b], ub = want[b] && n.present[a];
+      if (!ua && !ub) continue;
+      // dA/dphi: q of the cell-centred u (function_A_01.h:4-71)
+      {
+        R u[2], q[2], d[2];
+        u[0] = (n.ph[a] * n.cXc[b] - n.ph[b] * n.cXc[a]);
+        u[1] = (n.ph[a] * n.cYc[b] - n.ph[b] * n.cYc[a]);
+        rot2(p.rot[a][b], u, q);
+        dadq_of_q(p.dab[a][b], q, d);
+        R qab2 = (q[0] * q[0] + q[1] * q[1]);
+        R ac1 = ac_of_q(p.dab[a][b], q), ac2 = ac1 * ac1;
+        if (ua) {
+          R dq[2] = {n.cXc[b], n.cYc[b]}, rdq[2];
+          rot2(p.rot[a][b], dq, rdq);
+          R s1 = R(0), s2 = R(0);
+          s1 += q[0] * rdq[0];
+          s1 += q[1] * rdq[1];
+          s2 += d[0] * rdq[0];
+          s2 += d[1] * rdq[1];
+          s1 *= p.gamma[a][b] * ac2;
+          s2 *= p.gamma[a][b] * ac1 * qab2;
+          a1[a] += s1;
+          a2[a] += s2;
+        }
+        if (ub) {
+          R dq[2] = {n.cXc[a], n.cYc[a]}, rdq[2];
+          rot2(p.rot[b][a], dq, rdq);
+          R s1 = R(0), s2 = R(0);
+          s1 += (-q[0]) * rdq[0];
+          s1 += (-q[1]) * rdq[1];
+          s2 += (-d[0]) * rdq[0];
+          s2 += (-d[1]) * rdq[1];
+          s1 *= p.gamma[b][a] * ac2;
+          s2 *= p.gamma[b][a] * ac1 * qab2;
+          a1[b] += s1;
+          a2[b] += s2;
+        }
+      }
+      // div(dA/dgrad phi): the four faces (function_A_01.h:101-219)
+      FaceQ<R> F, Bk, Rt, L;
+      face_q(p, a, b, n.sXc[a] * n.gXc[b] - n.sXc[b] * n.gXc[a],
+             R(0.5) * (n.sXc[a] * (n.cYc[b] + n.cYf[b]) - n.sXc[b] * (n.cYc[a] + n.cYf[a])),
+             n.sXc[a] * n.gXc[b] - n.sXc[b] * n.gXc[a], F);
+      face_q(p, a, b, n.sXb[a] * n.gXb[b] - n.sXb[b] * n.gXb[a],
+             R(0.5) * (n.sXb[a] * (n.cYb[b] + n.cYc[b]) - n.sXb[b] * (n.cYb[a] + n.cYc[a])),
+             n.sXb[a] * n.gXb[b] - n.sXb[b] * n.gXb[a], Bk);
+      face_q(p, a, b, R(0.5) * (n.sYc[a] * (n.cXc[b] + n.cXr[b]) - n.sYc[b] * (n.cXc[a] + n.cXr[a])),
+             n.sYc[a] * n.gYc[b] - n.sYc[b] * n.gYc[a],
+             n.sYc[a] * n.gYc[b] - n.sYc[b] * n.gYc[a], Rt);
+      face_q(p, a, b, R(0.5) * (n.sYl[a] * (n.cXl[b] + n.cXc[b]) - n.sYl[b] * (n.cXl[a] + n.cXc[a])),
+             n.sYl[a] * n.gYl[b] - n.sYl[b] * n.gYl[a],
+             n.sYl[a] * n.gYl[b] - n.sYl[b] * n.gYl[a], L);
+      if (ua) {
+        R d1 = div_dx((F.ac2 * F.flux * (-n.sXc[b])) - (Bk.ac2 * Bk.flux * (-n.sXb[b])), p);
+        d1 += div_dy((Rt.ac2 * Rt.flux * (-n.sYc[b])) - (L.ac2 * L.flux * (-n.sYl[b])), p);
+        d1 *= p.gamma[a][b];
+        g1[a] += d1;
+        R d2 = div_dx(F.ac * F.q2 * F.rd[0] * (-n.sXc[b]) - Bk.ac * Bk.q2 * Bk.rd[0] * (-n.sXb[b]), p);
+        d2 += div_dy(Rt.ac * Rt.q2 * Rt.rd[1] * (-n.sYc[b]) - L.ac * L.q2 * L.rd[1] * (-n.sYl[b]), p);
+        d2 *= p.gamma[a][b];
+        g2[a] += d2;
+      }
+      if (ub) {
+        R d1 = div_dx((F.ac2 * (-F.flux) * (-n.sXc[a])) - (Bk.ac2 * (-Bk.flux) * (-n.sXb[a])), p);
+        d1 += div_dy((Rt.ac2 * (-Rt.flux) * (-n.sYc[a])) - (L.ac2 * (-L.flux) * (-n.sYl[a])), p);
+        d1 *= p.gamma[b][a];
+        g1[b] += d1;
+        R d2 = div_dx(F.ac * F.q2 * (-F.rd[0]) * (-n.sXc[a]) - Bk.ac * Bk.q2 * (-Bk.rd[0]) * (-n.sXb[a]), p);
+        d2 += div_dy(Rt.ac * Rt.q2 * (-Rt.rd[1]) * (-n.sYc[a]) - L.ac * L.q2 * (-L.rd[1]) * (-n.sYl[a]), p);
+        d2 *= p.gamma[b][a];
+        g2[b] += d2;
+      }
+    }
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    dA[a] = R(2.0) * (a1[a] + a2[a]);
+    dG[a] = (R(2.0) * (g1[a] + g2[a]) + R(0));
+  }
+}
+
 // dw/dphi_a  (functionW_01.h:6-39 obstacle, functionW_02.h:4-33 well)
 template <int P, class R>
 __device__ __forceinline__ R dwdphi(const DevParams<R>& p, const R* ph, const bool* act, int a) {
@@ -473,14 +577,20 @@ __device__ __forceinline__ void cell_dphi(const DevParams<R>& p, const Nbhd<P, R
     for (int b = 0; b < P; b++) psi[b] = psi_of_mu<C>(p, mu, b);
     R sl = R(0);
     int active = 0;
+    R dA[P], dG[P];
+    const bool pairs = p.anisotropy && p.pair_sym;
+    if (pairs) aniso_pairs<P>(p, n, act, dA, dG);
 #pragma unroll
     for (int a = 0; a < P; a++)
       if (act[a]) {
         R dps = R(0);
 #pragma unroll
         for (int b = 0; b < P; b++) dps += psi[b] * dhphi<P>(n.ph, b, a);
-        lam[a] = p.epsilon * (-dAdphi<P>(p, n, a) + divdAdgradphi<P>(p, n, a)) -
-                 p.inv_eps * dwdphi<P>(p, n.ph, act, a) - dps;
+        if (!pairs) {
+          dA[a] = dAdphi<P>(p, n, a);
+          dG[a] = divdAdgradphi<P>(p, n, a);
+        }
+        lam[a] = p.epsilon * (-dA[a] + dG[a]) - p.inv_eps * dwdphi<P>(p, n.ph, act, a) - dps;
         sl += lam[a];
         active++;
       }

@@ -77,6 +77,14 @@ static void fill_devparams(const gp_params &h, int rx, int ry, int pitch, long p
   d.well_at = (R)(0.5 / sqrt(2.0));
   d.diff_s = (R)h.diffusivity[0][0][0];
   d.diff_l = (R)h.diffusivity[h.nphases - 1][0][0];
+  d.pair_sym = 1;
+  for (int a = 0; a < P; a++)
+    for (int b = 0; b < P; b++) {
+      if (h.dab[a][b] != h.dab[b][a]) d.pair_sym = 0;
+      for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+          if (h.rot[a][b][i][j] != h.rot[b][a][i][j] || h.rot_inv[a][b][i][j] != h.rot_inv[b][a][i][j]) d.pair_sym = 0;
+    }
   for (int a = 0; a < P; a++) {
     for (int b = 0; b < P; b++) {
       d.gamma[a][b] = (R)h.gamma[a][b];
