@@ -323,7 +323,7 @@ static int adj_step(gp_adjoint *A, const void *phi_s, const void *mu_s) {
   if (rc) return rc;
   GP_HIP_CHECK(hipEventRecord(ep[0], c->stream));
   dim3 blk(BX, BY);
-  dim3 g_phase((c->ry - 4 + BX - 1) / BX, (c->rx - 4 + BY - 1) / BY);
+  dim3 g_phase = phase_grid(c->rx, c->ry);
   hipLaunchKernelGGL((k_phase<2, 1, R>), g_phase, blk, 0, c->stream, prm, phi, mu, (R *)c->dphi);
   const int MX = (int)c->hp.mesh_x, MY = (int)c->hp.mesh_y;
   dim3 ab(ABX, ABY);

@@ -556,20 +556,28 @@ __device__ __forceinline__ void project_simplex(const R* ph, const bool* act, R*
     }
 }
 
-// delta-phi of one cell  (calculate_divergence_phasefield.h:4-62)
-template <int P, int C, class R>
-__device__ __forceinline__ void cell_dphi(const DevParams<R>& p, const Nbhd<P, R>& n, const R* mu, R* dp) {
-  R divphi[P], lam[P], psi[P];
-  bool act[P];
-#pragma unroll
-  for (int a = 0; a < P; a++) {
-    divphi[a] = div_dx(n.gXc[a] - n.gXb[a], p);
-    divphi[a] += div_dy(n.gYc[a] - n.gYl[a], p);
-    act[a] = fabs(divphi[a]) > R(0);
-  }
+// Active phases of a cell: |div phi_a| > 0 (calculate_divergence_phasefield.h:9-16).
+// Returns whether any phase is active.
+template <int P, class R>
+__device__ __forceinline__ bool cell_active(const DevParams<R>& p, const Nbhd<P, R>& n, bool* act) {
   bool any = false;
 #pragma unroll
-  for (int a = 0; a < P; a++) any = any || act[a];
+  for (int a = 0; a < P; a++) {
+    R divphi = div_dx(n.gXc[a] - n.gXb[a], p);
+    divphi += div_dy(n.gYc[a] - n.gYl[a], p);
+    act[a] = fabs(divphi) > R(0);
+    any = any || act[a];
+  }
+  return any;
+}
+
+// delta-phi of one cell from its active set; with `pairs`, dA / dG hold the
+// anisotropic dA/dphi and div(dA/dgrad phi) of every active phase, else they
+// are evaluated here phase by phase  (calculate_divergence_phasefield.h:4-62)
+template <int P, int C, class R>
+__device__ __forceinline__ void cell_dphi_rest(const DevParams<R>& p, const Nbhd<P, R>& n, const R* mu,
+                                               const bool* act, bool any, bool pairs, R* dA, R* dG, R* dp) {
+  R lam[P], psi[P];
 #pragma unroll
   for (int a = 0; a < P; a++) dp[a] = R(0);
   if (any) {
@@ -577,9 +585,6 @@ __device__ __forceinline__ void cell_dphi(const DevParams<R>& p, const Nbhd<P, R
     for (int b = 0; b < P; b++) psi[b] = psi_of_mu<C>(p, mu, b);
     R sl = R(0);
     int active = 0;
-    R dA[P], dG[P];
-    const bool pairs = p.anisotropy && p.pair_sym;
-    if (pairs) aniso_pairs<P>(p, n, act, dA, dG);
 #pragma unroll
     for (int a = 0; a < P; a++)
       if (act[a]) {
@@ -602,6 +607,17 @@ __device__ __forceinline__ void cell_dphi(const DevParams<R>& p, const Nbhd<P, R
   }
   // the projection also runs where no phase is active: it lifts negative phi
   project_simplex<P>(n.ph, act, dp);
+}
+
+// delta-phi of one cell  (calculate_divergence_phasefield.h:4-62)
+template <int P, int C, class R>
+__device__ __forceinline__ void cell_dphi(const DevParams<R>& p, const Nbhd<P, R>& n, const R* mu, R* dp) {
+  bool act[P];
+  const bool any = cell_active<P>(p, n, act);
+  const bool pairs = p.anisotropy && p.pair_sym;
+  R dA[P], dG[P];
+  if (any && pairs) aniso_pairs<P>(p, n, act, dA, dG);
+  cell_dphi_rest<P, C>(p, n, mu, act, any, pairs, dA, dG, dp);
 }
 
 // ---------------------------------------------------------------- mu update

@@ -80,19 +80,18 @@ __device__ __forceinline__ void phase_cell(const DevParams<R>& p, const R* __res
   for (int a = 0; a < P; a++) dphi[idx(p, a, x, y)] = dp[a];
 }
 
-// mu and phi update of interior cell (x,y) with its ghost images; Kahan adds
-// the mu increment to mu + mulo (binary32).  Returns the one-hot phase of the
-// updated phi (-1 if not one-hot).
+// mu and phi update of interior cell (x,y) from its 3x3 phi block v, the mu of
+// its five-point stencil, its delta-phi and the antitrapping fluxes of its four
+// faces; writes the cell and its ghost images.  Kahan adds the mu increment to
+// mu + mulo (binary32).  Returns the one-hot phase of the updated phi (-1 if
+// not one-hot).
 template <int P, int C, class R, bool Kahan>
-__device__ __forceinline__ int conc_cell(const DevParams<R>& p, const R* __restrict__ phi, const R* __restrict__ mu,
-                                         const R* __restrict__ dphi, const R* __restrict__ mulo, R* __restrict__ phi_o,
-                                         R* __restrict__ mu_o, R* __restrict__ mulo_o, int x, int y) {
-  R v[P][3][3];
-  load3x3<P>(p, phi, x, y, v);
-  Nbhd<P, R> n;
-  make_nbhd<P>(p, v, n);
+__device__ __forceinline__ int conc_apply(const DevParams<R>& p, const R (&v)[P][3][3], const R* mc, const R* mf,
+                                          const R* mb, const R* mr, const R* ml, const R* dc, const R* jf,
+                                          const R* jb, const R* jr, const R* jl, const R* __restrict__ mulo,
+                                          R* __restrict__ phi_o, R* __restrict__ mu_o, R* __restrict__ mulo_o, int x,
+                                          int y) {
   R phc[P], phf[P], phb[P], phr[P], phl[P];
-  R dc[P], df[P], db[P], dr[P], dl[P];
 #pragma unroll
   for (int a = 0; a < P; a++) {
     phc[a] = v[a][1][1];
@@ -100,27 +99,7 @@ __device__ __forceinline__ int conc_cell(const DevParams<R>& p, const R* __restr
     phb[a] = v[a][0][1];
     phr[a] = v[a][1][2];
     phl[a] = v[a][1][0];
-    dc[a] = dphi[idx(p, a, x, y)];
-    df[a] = dphi[idx(p, a, x + 1, y)];
-    db[a] = dphi[idx(p, a, x - 1, y)];
-    dr[a] = dphi[idx(p, a, x, y + 1)];
-    dl[a] = dphi[idx(p, a, x, y - 1)];
   }
-  R mc[C], mf[C], mb[C], mr[C], ml[C];
-#pragma unroll
-  for (int k = 0; k < C; k++) {
-    mc[k] = mu[idx(p, k, x, y)];
-    mf[k] = mu[idx(p, k, x + 1, y)];
-    mb[k] = mu[idx(p, k, x - 1, y)];
-    mr[k] = mu[idx(p, k, x, y + 1)];
-    ml[k] = mu[idx(p, k, x, y - 1)];
-  }
-  // antitrapping on the four faces
-  R jf[C], jb[C], jr[C], jl[C];
-  face_jat<P, C>(p, n.gXc, n.cYc, n.cYf, phc, phf, mc, mf, dc, df, true, jf);
-  face_jat<P, C>(p, n.gXb, n.cYb, n.cYc, phb, phc, mb, mc, db, dc, true, jb);
-  face_jat<P, C>(p, n.gYc, n.cXc, n.cXr, phc, phr, mc, mr, dc, dr, y != p.yskip, jr);
-  face_jat<P, C>(p, n.gYl, n.cXl, n.cXc, phl, phc, ml, mc, dl, dc, (y - 1) != p.yskip, jl);
   // face mobilities and mu gradients (calculate_gradients.h:148-165)
   R Dc[C][C], Df[C][C], Db[C][C], Dr[C][C], Dl[C][C];
   cell_D<P, C>(p, phc, Dc);
@@ -170,6 +149,52 @@ __device__ __forceinline__ int conc_cell(const DevParams<R>& p, const R* __restr
   return nz == 0 ? a1 : -1;
 }
 
+// mu and phi update of interior cell (x,y) with its ghost images (all four
+// face fluxes computed here).  Returns the one-hot phase as conc_apply.
+template <int P, int C, class R, bool Kahan>
+__device__ __forceinline__ int conc_cell(const DevParams<R>& p, const R* __restrict__ phi, const R* __restrict__ mu,
+                                         const R* __restrict__ dphi, const R* __restrict__ mulo, R* __restrict__ phi_o,
+                                         R* __restrict__ mu_o, R* __restrict__ mulo_o, int x, int y) {
+  R v[P][3][3];
+  load3x3<P>(p, phi, x, y, v);
+  Nbhd<P, R> n;
+  make_nbhd<P>(p, v, n);
+  R phc[P], phf[P], phb[P], phr[P], phl[P];
+  R dc[P], df[P], db[P], dr[P], dl[P];
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    phc[a] = v[a][1][1];
+    phf[a] = v[a][2][1];
+    phb[a] = v[a][0][1];
+    phr[a] = v[a][1][2];
+    phl[a] = v[a][1][0];
+    dc[a] = dphi[idx(p, a, x, y)];
+    df[a] = dphi[idx(p, a, x + 1, y)];
+    db[a] = dphi[idx(p, a, x - 1, y)];
+    dr[a] = dphi[idx(p, a, x, y + 1)];
+    dl[a] = dphi[idx(p, a, x, y - 1)];
+  }
+  R mc[C], mf[C], mb[C], mr[C], ml[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    mc[k] = mu[idx(p, k, x, y)];
+    mf[k] = mu[idx(p, k, x + 1, y)];
+    mb[k] = mu[idx(p, k, x - 1, y)];
+    mr[k] = mu[idx(p, k, x, y + 1)];
+    ml[k] = mu[idx(p, k, x, y - 1)];
+  }
+  // antitrapping on the four faces
+  R jf[C], jb[C], jr[C], jl[C];
+  face_jat<P, C>(p, n.gXc, n.cYc, n.cYf, phc, phf, mc, mf, dc, df, true, jf);
+  face_jat<P, C>(p, n.gXb, n.cYb, n.cYc, phb, phc, mb, mc, db, dc, true, jb);
+  face_jat<P, C>(p, n.gYc, n.cXc, n.cXr, phc, phr, mc, mr, dc, dr, y != p.yskip, jr);
+  face_jat<P, C>(p, n.gYl, n.cXl, n.cXc, phl, phc, ml, mc, dl, dc, (y - 1) != p.yskip, jl);
+  return conc_apply<P, C, R, Kahan>(p, v, mc, mf, mb, mr, ml, dc, jf, jb, jr, jl, mulo, phi_o, mu_o, mulo_o, x, y);
+}
+
+// delta-phi on [2, rows-3]^2, one cell per thread
+__host__ __device__ inline dim3 phase_grid(int rx, int ry) { return dim3((ry - 4 + BX - 1) / BX, (rx - 4 + BY - 1) / BY); }
+
 template <int P, int C, class R>
 __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                    const R* __restrict__ mu, R* __restrict__ dphi) {
@@ -190,6 +215,107 @@ __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_conc(const DevParams<R>* _
   const int x = 3 + blockIdx.y * BY + threadIdx.y;
   if (x > p.mesh_x + 2 || y > p.mesh_y + 2) return;
   conc_cell<P, C, R, !Ops<R>::Exact>(p, phi, mu, dphi, mulo, phi_o, mu_o, mulo_o, x, y);
+}
+
+// Face-sharing variant of k_conc.  The antitrapping flux of a face is the same
+// function of the same operands whichever of its two cells evaluates it
+// (face_jat on the pair (lower, upper)), so a face need not be evaluated
+// twice: a wave holds 64 consecutive y columns, starting one column before the
+// 63 cells it updates, and marches KM rows along x.  Each lane evaluates the
+// +x and +y faces of its cell; the -y flux comes from the lane below, the -x
+// flux of row i > 0 is the +x flux of row i-1 and the first row evaluates its
+// own.  That is 1 + 1/KM + 64/63 face evaluations per cell instead of 4.
+// KM = 1 is the fastest at 1024^2 with four phases (on 256 CUs more waves in
+// flight outweigh the shared x faces; KM = 2, 3, 4 were 2.5%, 4% and 8.5%
+// slower in binary64).  Lanes past the last column read a clamped column and
+// write nothing.
+#ifndef GP_KM
+#define GP_KM 1
+#endif
+constexpr int KM = GP_KM;  // rows per wave
+constexpr int MW = 63;    // cells per wave along y
+
+template <int P, int C, class R>
+__global__ void __launch_bounds__(BX * BY) GP_HEAVY k_conc_march(const DevParams<R>* __restrict__ prm,
+                                                              const R* __restrict__ phi, const R* __restrict__ mu,
+                                                              const R* __restrict__ dphi, const R* __restrict__ mulo,
+                                                              R* __restrict__ phi_o, R* __restrict__ mu_o,
+                                                              R* __restrict__ mulo_o) {
+  const DevParams<R>& p = *prm;
+  const int lane = threadIdx.x;
+  const int y = 2 + blockIdx.x * MW + lane;  // lane 0 sits one column before the wave's cells
+  const int yc = min(y, p.mesh_y + 2);
+  const bool cell = lane > 0 && y <= p.mesh_y + 2;
+  const int x0 = 3 + (blockIdx.y * BY + threadIdx.y) * KM;
+  if (x0 > p.mesh_x + 2) return;  // uniform over the wave
+  R v[P][3][3];
+  load3x3<P>(p, phi, x0, yc, v);
+  R dc[P], jb[C];
+#pragma unroll
+  for (int a = 0; a < P; a++) dc[a] = dphi[idx(p, a, x0, yc)];
+  {
+    Nbhd<P, R> n;
+    make_nbhd<P>(p, v, n);
+    R phb[P], phc[P], db[P], mb[C], mc[C];
+#pragma unroll
+    for (int a = 0; a < P; a++) {
+      phb[a] = v[a][0][1];
+      phc[a] = v[a][1][1];
+      db[a] = dphi[idx(p, a, x0 - 1, yc)];
+    }
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      mb[k] = mu[idx(p, k, x0 - 1, yc)];
+      mc[k] = mu[idx(p, k, x0, yc)];
+    }
+    face_jat<P, C>(p, n.gXb, n.cYb, n.cYc, phb, phc, mb, mc, db, dc, true, jb);
+  }
+  for (int i = 0; i < KM; i++) {
+    const int x = x0 + i;
+    if (x > p.mesh_x + 2) break;  // uniform over the wave
+    if (i > 0) {
+#pragma unroll
+      for (int a = 0; a < P; a++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          v[a][0][j] = v[a][1][j];
+          v[a][1][j] = v[a][2][j];
+          v[a][2][j] = phi[idx(p, a, x + 1, yc + j - 1)];
+        }
+    }
+    Nbhd<P, R> n;
+    make_nbhd<P>(p, v, n);
+    R phc[P], phf[P], phr[P], df[P], dr[P];
+#pragma unroll
+    for (int a = 0; a < P; a++) {
+      phc[a] = v[a][1][1];
+      phf[a] = v[a][2][1];
+      phr[a] = v[a][1][2];
+      df[a] = dphi[idx(p, a, x + 1, yc)];
+      dr[a] = dphi[idx(p, a, x, yc + 1)];
+    }
+    R mc[C], mf[C], mb[C], mr[C], ml[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      mc[k] = mu[idx(p, k, x, yc)];
+      mf[k] = mu[idx(p, k, x + 1, yc)];
+      mb[k] = mu[idx(p, k, x - 1, yc)];
+      mr[k] = mu[idx(p, k, x, yc + 1)];
+      ml[k] = mu[idx(p, k, x, yc - 1)];
+    }
+    R jf[C], jr[C], jl[C];
+    face_jat<P, C>(p, n.gXc, n.cYc, n.cYf, phc, phf, mc, mf, dc, df, true, jf);
+    face_jat<P, C>(p, n.gYc, n.cXc, n.cXr, phc, phr, mc, mr, dc, dr, yc != p.yskip, jr);
+#pragma unroll
+    for (int k = 0; k < C; k++) jl[k] = __shfl_up(jr[k], 1);
+    if (cell)
+      conc_apply<P, C, R, !Ops<R>::Exact>(p, v, mc, mf, mb, mr, ml, dc, jf, jb, jr, jl, mulo, phi_o, mu_o, mulo_o, x,
+                                          y);
+#pragma unroll
+    for (int k = 0; k < C; k++) jb[k] = jf[k];
+#pragma unroll
+    for (int a = 0; a < P; a++) dc[a] = df[a];
+  }
 }
 
 // Fill the ghost layers of a freshly loaded state: every interior cell copies
