@@ -29,6 +29,21 @@ namespace gpk {
 constexpr int BX = 64;  // threads along y (contiguous)
 constexpr int BY = 4;   // threads along x
 
+// XCD-aware block order.  Workgroups are dispatched to the 8 XCDs round-robin
+// by linear id, and each XCD has its own L2, so with the natural order the
+// stencil halo rows between neighbouring blocks are fetched into different
+// L2s.  blk_xy maps the linear id L (XCD L % 8, its (L / 8)-th block) to a
+// logical block of a contiguous range per XCD, in dispatch order, so that an
+// XCD sweeps its own band of rows and its L2 holds the halo it re-reads.  The
+// map is a bijection of [0, gridDim.x * gridDim.y).
+__device__ __forceinline__ void blk_xy(int& bx, int& by) {
+  const int n = gridDim.x * gridDim.y, L = blockIdx.y * gridDim.x + blockIdx.x;
+  const int q = n >> 3, r = n & 7, xcd = L & 7, s = L >> 3;
+  const int l = xcd * q + min(xcd, r) + s;
+  bx = l % gridDim.x;
+  by = l / gridDim.x;
+}
+
 template <class R>
 __device__ __forceinline__ size_t idx(const DevParams<R>& p, int a, int x, int y) {
   return (size_t)a * p.plane + (size_t)x * p.pitch + y;
@@ -199,8 +214,10 @@ template <int P, int C, class R>
 __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                    const R* __restrict__ mu, R* __restrict__ dphi) {
   const DevParams<R>& p = *prm;
-  const int y = 2 + blockIdx.x * BX + threadIdx.x;
-  const int x = 2 + blockIdx.y * BY + threadIdx.y;
+  int bx, by;
+  blk_xy(bx, by);
+  const int y = 2 + bx * BX + threadIdx.x;
+  const int x = 2 + by * BY + threadIdx.y;
   if (x > p.rx - 3 || y > p.ry - 3) return;
   phase_cell<P, C>(p, phi, mu, dphi, x, y);
 }
@@ -243,10 +260,12 @@ __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_conc_march(const DevParams
                                                               R* __restrict__ mulo_o) {
   const DevParams<R>& p = *prm;
   const int lane = threadIdx.x;
-  const int y = 2 + blockIdx.x * MW + lane;  // lane 0 sits one column before the wave's cells
+  int bx, by;
+  blk_xy(bx, by);
+  const int y = 2 + bx * MW + lane;  // lane 0 sits one column before the wave's cells
   const int yc = min(y, p.mesh_y + 2);
   const bool cell = lane > 0 && y <= p.mesh_y + 2;
-  const int x0 = 3 + (blockIdx.y * BY + threadIdx.y) * KM;
+  const int x0 = 3 + (by * BY + threadIdx.y) * KM;
   if (x0 > p.mesh_x + 2) return;  // uniform over the wave
   R v[P][3][3];
   load3x3<P>(p, phi, x0, yc, v);
@@ -316,6 +335,76 @@ __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_conc_march(const DevParams
 #pragma unroll
     for (int a = 0; a < P; a++) dc[a] = df[a];
   }
+}
+
+// k_conc_march with KM = 1 and the x faces shared through LDS: the four waves
+// of a block hold four consecutive x rows; each wave evaluates the +x and +y
+// faces of its row, the first wave also its -x face, and the -x flux of row
+// w > 0 is read from row w-1 after a barrier.  1 + 1/BY + 64/63 face
+// evaluations per cell.
+template <int P, int C, class R>
+__global__ void __launch_bounds__(BX * BY) GP_HEAVY k_conc_lds(const DevParams<R>* __restrict__ prm,
+                                                            const R* __restrict__ phi, const R* __restrict__ mu,
+                                                            const R* __restrict__ dphi, const R* __restrict__ mulo,
+                                                            R* __restrict__ phi_o, R* __restrict__ mu_o,
+                                                            R* __restrict__ mulo_o) {
+  __shared__ R jx[C][BY][BX];
+  const DevParams<R>& p = *prm;
+  const int lane = threadIdx.x, w = threadIdx.y;
+  int bx, by;
+  blk_xy(bx, by);
+  const int y = 2 + bx * MW + lane;  // lane 0 sits one column before the wave's cells
+  const int yc = min(y, p.mesh_y + 2);
+  const int x = 3 + by * BY + w;
+  const int xc = min(x, p.mesh_x + 2);       // waves past the last row compute a copy and write nothing
+  const bool cell = x <= p.mesh_x + 2 && lane > 0 && y <= p.mesh_y + 2;
+  R v[P][3][3];
+  load3x3<P>(p, phi, xc, yc, v);
+  Nbhd<P, R> n;
+  make_nbhd<P>(p, v, n);
+  R phc[P], phf[P], phr[P], dc[P], df[P], dr[P];
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    phc[a] = v[a][1][1];
+    phf[a] = v[a][2][1];
+    phr[a] = v[a][1][2];
+    dc[a] = dphi[idx(p, a, xc, yc)];
+    df[a] = dphi[idx(p, a, xc + 1, yc)];
+    dr[a] = dphi[idx(p, a, xc, yc + 1)];
+  }
+  R mc[C], mf[C], mb[C], mr[C], ml[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    mc[k] = mu[idx(p, k, xc, yc)];
+    mf[k] = mu[idx(p, k, xc + 1, yc)];
+    mb[k] = mu[idx(p, k, xc - 1, yc)];
+    mr[k] = mu[idx(p, k, xc, yc + 1)];
+    ml[k] = mu[idx(p, k, xc, yc - 1)];
+  }
+  R jf[C], jb[C], jr[C], jl[C];
+  face_jat<P, C>(p, n.gXc, n.cYc, n.cYf, phc, phf, mc, mf, dc, df, true, jf);
+  face_jat<P, C>(p, n.gYc, n.cXc, n.cXr, phc, phr, mc, mr, dc, dr, yc != p.yskip, jr);
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    jl[k] = __shfl_up(jr[k], 1);
+    jx[k][w][lane] = jf[k];
+  }
+  if (w == 0) {  // uniform over the wave
+    R phb[P], db[P];
+#pragma unroll
+    for (int a = 0; a < P; a++) {
+      phb[a] = v[a][0][1];
+      db[a] = dphi[idx(p, a, xc - 1, yc)];
+    }
+    face_jat<P, C>(p, n.gXb, n.cYb, n.cYc, phb, phc, mb, mc, db, dc, true, jb);
+  }
+  __syncthreads();
+  if (w > 0) {
+#pragma unroll
+    for (int k = 0; k < C; k++) jb[k] = jx[k][w - 1][lane];
+  }
+  if (cell)
+    conc_apply<P, C, R, !Ops<R>::Exact>(p, v, mc, mf, mb, mr, ml, dc, jf, jb, jr, jl, mulo, phi_o, mu_o, mulo_o, x, y);
 }
 
 // Fill the ghost layers of a freshly loaded state: every interior cell copies
