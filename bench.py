@@ -73,7 +73,8 @@ def adjoint_report(a):
     """Dendrite (2 phases, binary, 512^2, anisotropy 0.02): gradients of the phi
     mismatch to a target made with perturbed (dab, Tau, GAMMA, D_liquid), by the
     adjoint with all states kept and with 20 snapshots, against central
-    differences (h = 1e-6 relative); time of the adjoint run over the forward."""
+    differences (h = 1e-6 relative); time of the adjoint run over the fastest
+    forward run of the same steps (automatic, two-pass or sparse fused path)."""
     import copy
     import numpy as np
     from eshwar_dipan_amd import gpstep, workloads
@@ -112,15 +113,22 @@ def adjoint_report(a):
     out = {"metric": "adjoint / forward time; gradient error vs central differences", "steps": steps,
            "config": {"workload": desc, "mesh_x": p.mesh_x, "mesh_y": p.mesh_y}, "runs": []}
     for dtype in ("f64", "f32"):
+        # forward time-to-solution of the same run, fastest of the automatic,
+        # two-pass and sparse fused paths (graphs captured outside the timing)
+        t_paths = {}
+        for path in (-1, 1, 0):
+            s = gpstep.Solver(p, dtype, path=path)
+            s.set_state(phi0, mu0)
+            s.step(64)
+            s.set_state(phi0, mu0)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step(steps)
+            s.synchronize()
+            t_paths[path] = time.perf_counter() - t0
+            s.close()
+        t_fwd = min(t_paths.values())
         s = gpstep.Solver(p, dtype)
-        s.set_state(phi0, mu0)
-        s.step(64)  # graph capture outside the timing
-        s.set_state(phi0, mu0)
-        s.synchronize()
-        t0 = time.perf_counter()
-        s.step(steps)
-        s.synchronize()
-        t_fwd = time.perf_counter() - t0
         for nsnap in (0, 20):
             adj = gpstep.Adjoint(s, steps, nsnap)
             t0 = time.perf_counter()
@@ -130,6 +138,8 @@ def adjoint_report(a):
             adj.close()
             out["runs"].append({"dtype": dtype, "snapshots": nsnap or steps, "loss": loss, "grad": grad,
                                 "forward_s": t_fwd, "adjoint_s": t_adj, "ratio": t_adj / t_fwd,
+                                "forward_s_by_path": {"automatic": t_paths[-1], "two-pass": t_paths[1],
+                                                      "sparse fused": t_paths[0]},
                                 "adjoint_forward_steps": nfwd, "device_fwd_ms": f_ms, "device_rev_ms": r_ms})
             print(f"{dtype} nsnap={nsnap}: adjoint/forward {t_adj / t_fwd:.2f}", file=sys.stderr, flush=True)
         s.close()

@@ -212,6 +212,8 @@ struct gp_adjoint {
   long nsteps;
   int nslots;
   size_t slot_bytes;     // bytes of one saved state (phi, mu and mu low words, padded planes)
+  bool direct;           // one slot per step: the forward steps slot k into slot k+1 and keeps delta-phi
+  size_t dphi_off;       // byte offset of delta-phi in a slot (direct)
   std::vector<void *> slots;
   void *cg, *cf;         // partial planes of the update and phase reverses
   void *lam[2][2];       // [buffer][0 = phi (2 planes), 1 = mu]
@@ -272,17 +274,34 @@ static int adj_restore(gp_adjoint *A, int slot) {
   return GP_OK;
 }
 
+// n forward steps of the context with the two-pass kernels (the fastest path
+// for the two-phase runs; every path gives the same binary64 states)
+template <class R>
+static void adj_two_pass(gp_ctx *c, long n) {
+  for (long t = 0; t < n; t++) {
+    gpk::launch_two_pass<2, 1, R>((const gpk::DevParams<R> *)c->dprm, c->rx, c->ry, (int)c->hp.mesh_x,
+                                  (int)c->hp.mesh_y, c->stream, (const R *)c->phi[c->cur], (const R *)c->mu[c->cur],
+                                  (R *)c->dphi, (const R *)c->mulo[c->cur], (R *)c->phi[c->cur ^ 1],
+                                  (R *)c->mu[c->cur ^ 1], (R *)c->mulo[c->cur ^ 1]);
+    c->cur ^= 1;
+  }
+  c->flags_valid = 0;
+  c->stepped = 1;
+}
+
 static int adj_advance(gp_adjoint *A, long n) {
   if (n <= 0) return GP_OK;
   hipEvent_t *e;
   int rc = ev_pair(A, 0, &e);
   if (rc) return rc;
   GP_HIP_CHECK(hipEventRecord(e[0], A->c->stream));
-  rc = gp_step_impl(A->c, n, false);
+  if (A->c->dtype == GP_F64) adj_two_pass<double>(A->c, n);
+  else adj_two_pass<float>(A->c, n);
+  GP_HIP_CHECK(hipGetLastError());
   GP_HIP_CHECK(hipEventRecord(e[1], A->c->stream));
   A->fwd_steps += n;
   A->cur_step += n;
-  return rc;
+  return GP_OK;
 }
 
 // the loss and its seed from the state after the last step
@@ -313,7 +332,7 @@ static int adj_seed(gp_adjoint *A) {
 // one reverse step from the state S_k at (phi, mu) (the context's buffers or
 // a snapshot slot, same padded layout): lam <- J_k^T lam
 template <class R>
-static int adj_step(gp_adjoint *A, const void *phi_s, const void *mu_s) {
+static int adj_step(gp_adjoint *A, const void *phi_s, const void *mu_s, const void *dphi_s) {
   gp_ctx *c = A->c;
   using namespace gpk;
   const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
@@ -322,16 +341,19 @@ static int adj_step(gp_adjoint *A, const void *phi_s, const void *mu_s) {
   int rc = ev_pair(A, 1, &ep);
   if (rc) return rc;
   GP_HIP_CHECK(hipEventRecord(ep[0], c->stream));
-  dim3 blk(BX, BY);
-  dim3 g_phase = phase_grid(c->rx, c->ry);
-  hipLaunchKernelGGL((k_phase<2, 1, R>), g_phase, blk, 0, c->stream, prm, phi, mu, (R *)c->dphi);
+  const R *dphi = (const R *)dphi_s;
+  if (!dphi) {  // delta-phi of S_k not kept: recompute it
+    hipLaunchKernelGGL((k_phase<2, 1, R>), phase_grid(c->rx, c->ry), dim3(BX, BY), 0, c->stream, prm, phi, mu,
+                       (R *)c->dphi);
+    dphi = (const R *)c->dphi;
+  }
   const int MX = (int)c->hp.mesh_x, MY = (int)c->hp.mesh_y;
   dim3 ab(ABX, ABY);
   dim3 g_in((MY + ABX - 1) / ABX, (MX + ABY - 1) / ABY);
   dim3 g_ring((c->ry - 4 + ABX - 1) / ABX, (c->rx - 4 + ABY - 1) / ABY);
   const int li = A->lcur, lo = A->lcur ^ 1;
   const int nb_in = (int)(g_in.x * g_in.y), nb_ring = (int)(g_ring.x * g_ring.y);
-  hipLaunchKernelGGL((k_adj_conc<R>), g_in, ab, 0, c->stream, prm, phi, mu, (const R *)c->dphi,
+  hipLaunchKernelGGL((k_adj_conc<R>), g_in, ab, 0, c->stream, prm, phi, mu, dphi,
                      (const R *)A->lam[li][0], (const R *)A->lam[li][1], (R *)A->cg, A->part);
   hipLaunchKernelGGL((k_adj_phase<R>), g_ring, ab, 0, c->stream, prm, phi, mu, (const R *)A->cg, (R *)A->cf,
                      A->part + (size_t)nb_in * ADJ_NPAR);
@@ -355,7 +377,47 @@ static int adj_step_any(gp_adjoint *A, int sl, long k) {
   } else if (A->cur_step != k) {
     return GP_E_ARG;
   }
-  return c->dtype == GP_F64 ? adj_step<double>(A, phi, mu) : adj_step<float>(A, phi, mu);
+  return c->dtype == GP_F64 ? adj_step<double>(A, phi, mu, nullptr) : adj_step<float>(A, phi, mu, nullptr);
+}
+
+// Direct mode: step S_k in slot k into slot k+1 with the two-pass kernels,
+// keeping delta-phi(S_k) in slot k (no copies, no recomputation in reverse).
+template <class R>
+static int adj_forward_direct(gp_adjoint *A) {
+  gp_ctx *c = A->c;
+  const size_t pb = (size_t)c->plane * c->esize;
+  hipEvent_t *e;
+  int rc = ev_pair(A, 0, &e);
+  if (rc) return rc;
+  GP_HIP_CHECK(hipEventRecord(e[0], c->stream));
+  for (long k = 0; k < A->nsteps; k++) {
+    char *s0 = (char *)A->slots[k], *s1 = (char *)A->slots[k + 1];
+    const R *mulo = c->mulo[0] ? (const R *)(s0 + pb * (c->P + c->C)) : nullptr;
+    R *mulo_o = c->mulo[0] ? (R *)(s1 + pb * (c->P + c->C)) : nullptr;
+    gpk::launch_two_pass<2, 1, R>((const gpk::DevParams<R> *)c->dprm, c->rx, c->ry, (int)c->hp.mesh_x,
+                                  (int)c->hp.mesh_y, c->stream, (const R *)s0, (const R *)(s0 + pb * c->P),
+                                  (R *)(s0 + A->dphi_off), mulo, (R *)s1, (R *)(s1 + pb * c->P), mulo_o);
+    A->slot_step[k + 1] = k + 1;
+  }
+  GP_HIP_CHECK(hipGetLastError());
+  GP_HIP_CHECK(hipEventRecord(e[1], c->stream));
+  A->fwd_steps += A->nsteps;
+  return GP_OK;
+}
+
+template <class R>
+static int adj_run_direct(gp_adjoint *A) {
+  gp_ctx *c = A->c;
+  const size_t pb = (size_t)c->plane * c->esize;
+  int rc;
+  if ((rc = adj_forward_direct<R>(A))) return rc;
+  if ((rc = adj_restore(A, (int)A->nsteps))) return rc;  // the context holds S_n for the seed
+  if ((rc = adj_seed<R>(A))) return rc;
+  for (long k = A->nsteps - 1; k >= 0; k--) {
+    const char *sk = (const char *)A->slots[k];
+    if ((rc = adj_step<R>(A, sk, sk + pb * c->P, sk + A->dphi_off))) return rc;
+  }
+  return GP_OK;
 }
 
 static long binom(int s, int t) {
@@ -431,6 +493,9 @@ int gp_adjoint_create(gp_ctx *c, long nsteps, int nsnap, gp_adjoint **out) {
   A->nslots = 1 + (int)std::min<long>(nsnap ? nsnap : nsteps, nsteps);
   const size_t pb = (size_t)c->plane * c->esize;
   A->slot_bytes = pb * (c->P + c->C + (c->mulo[0] ? c->C : 0));
+  A->direct = A->nslots == nsteps + 1;
+  A->dphi_off = A->slot_bytes;
+  if (A->direct) A->slot_bytes += pb * c->P;
   bool ok = true;
   A->slots.assign(A->nslots, nullptr);
   for (int i = 0; i < A->nslots && ok; i++) ok = hipMalloc(&A->slots[i], A->slot_bytes) == hipSuccess;
@@ -482,9 +547,13 @@ int gp_adjoint_run(gp_adjoint *A, const double *phi0, const double *mu0, const d
   A->target = phi_target;
   GP_HIP_CHECK(hipMemset(A->acc, 0, sizeof(double) * gpk::ADJ_NPAR));
   if ((rc = adj_save(A, 0))) return rc;
-  std::vector<int> pool;
-  for (int i = A->nslots - 1; i >= 1; i--) pool.push_back(i);
-  rc = adj_reverse(A, 0, A->nsteps, 0, pool);
+  if (A->direct) {
+    rc = c->dtype == GP_F64 ? adj_run_direct<double>(A) : adj_run_direct<float>(A);
+  } else {
+    std::vector<int> pool;
+    for (int i = A->nslots - 1; i >= 1; i--) pool.push_back(i);
+    rc = adj_reverse(A, 0, A->nsteps, 0, pool);
+  }
   if (rc) return rc;
   GP_HIP_CHECK(hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < A->ev_kind.size(); i++) {

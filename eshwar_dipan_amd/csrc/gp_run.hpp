@@ -221,22 +221,11 @@ int run_steps(gp_ctx *c, long nsteps) {
     return run_sparse<P, C, R, 0>(c, nsteps);
   }
   const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
-  dim3 blk(BX, BY);
-  dim3 g_phase = phase_grid(c->rx, c->ry);
-  // binary64: x faces shared through LDS (3% faster at 1024^2, P=4); binary32:
-  // the barrier costs more than the face it saves (1% slower), one row per wave
-  constexpr bool lds = Ops<R>::Exact;
-  dim3 g_conc((c->hp.mesh_y + MW - 1) / MW, (c->hp.mesh_x + BY * (lds ? 1 : KM) - 1) / (BY * (lds ? 1 : KM)));
   for (long t = 0; t < nsteps; t++) {
-    const R *phi = (const R *)c->phi[c->cur], *mu = (const R *)c->mu[c->cur];
-    R *phi_o = (R *)c->phi[c->cur ^ 1], *mu_o = (R *)c->mu[c->cur ^ 1];
-    hipLaunchKernelGGL((k_phase<P, C, R>), g_phase, blk, 0, c->stream, prm, phi, mu, (R *)c->dphi);
-    if constexpr (lds)
-      hipLaunchKernelGGL((k_conc_lds<P, C, R>), g_conc, blk, 0, c->stream, prm, phi, mu, (const R *)c->dphi,
-                         (const R *)c->mulo[c->cur], phi_o, mu_o, (R *)c->mulo[c->cur ^ 1]);
-    else
-      hipLaunchKernelGGL((k_conc_march<P, C, R>), g_conc, blk, 0, c->stream, prm, phi, mu, (const R *)c->dphi,
-                         (const R *)c->mulo[c->cur], phi_o, mu_o, (R *)c->mulo[c->cur ^ 1]);
+    launch_two_pass<P, C, R>(prm, c->rx, c->ry, (int)c->hp.mesh_x, (int)c->hp.mesh_y, c->stream,
+                             (const R *)c->phi[c->cur], (const R *)c->mu[c->cur], (R *)c->dphi,
+                             (const R *)c->mulo[c->cur], (R *)c->phi[c->cur ^ 1], (R *)c->mu[c->cur ^ 1],
+                             (R *)c->mulo[c->cur ^ 1]);
     c->cur ^= 1;
   }
   c->flags_valid = 0;

@@ -222,9 +222,12 @@ int  gp_ensemble_synchronize(gp_ensemble *e);
  * BINARY), with respect to the anisotropy strength dab[0][1], the pair
  * relaxation constant Tau[0][1], the interface energy GAMMA[0][1] and the
  * liquid diffusivity, and optionally to the initial state.  The forward steps
- * run through the context's own kernels; the states are kept in nsnap snapshot
- * slots of device memory with binomial checkpointing (nsnap = 0: one slot per
- * step, no recomputation).  Parameter gradients are accumulated in binary64.
+ * run the context's two-pass kernels (the same binary64 states as every path);
+ * the states are kept in nsnap snapshot slots of device memory with binomial
+ * checkpointing (nsnap = 0, or nsnap >= nsteps: one slot per step holding the
+ * state and its delta-phi, no recomputation).  Parameter gradients are
+ * accumulated in binary64.  On return the context holds the state after the
+ * last recomputed step, not necessarily the final one.
  * No reference counterpart: the serial solver has no adjoint.
  */
 typedef struct gp_adjoint gp_adjoint;
