@@ -14,16 +14,50 @@
 
 namespace gpk {
 
-// planes of the per-direction partials
-constexpr int CG_PHI = 0;   // 9 directions x 2 phases
-constexpr int CG_MU = 18;   // 5 directions
-constexpr int CG_DP = 23;   // 5 directions x 2 phases
-constexpr int CG_N = 33;
-constexpr int CF_PHI = 0;   // 9 x 2
-constexpr int CF_MU = 18;   // own cell
-constexpr int CF_N = 19;
-
 constexpr int ABX = 64, ABY = 4;
+
+// The reverse kernels gather the partials of a tile's cells in LDS: a tile of
+// ABY x ABX cells scatters into its footprint of FPX x FPY cells (its 3x3
+// stencils), one stencil direction at a time (for a fixed direction the cells
+// hit distinct targets, so the sums need no atomics and have a fixed order),
+// and writes the footprint sums.  A cell's total is then the sum over the (at
+// most four) footprints that hold it, in ascending tile order.
+constexpr int FPX = ABY + 2, FPY = ABX + 2, FPN = FPX * FPY;
+constexpr int FC_N = 5;  // update reverse: phi_0, phi_1, mu, delta-phi_0, delta-phi_1
+constexpr int FF_N = 3;  // phase reverse: phi_0, phi_1, mu
+
+// Tiles of origin o + s*b and extent e (b < n) whose range holds x, ascending.
+__device__ __forceinline__ int fp_tiles(int x, int o, int s, int e, int n, int* b) {
+  int k = 0;
+  if (x < o) return 0;
+  const int hi = (x - o) / s;
+  for (int c = hi - 1; c <= hi; c++)
+    if (c >= 0 && c < n && x <= o + s * c + e - 1) b[k++] = c;
+  return k;
+}
+
+// Sum of values v0 .. v0+nv-1 at cell (x, y) over the footprints of tiles with
+// cell origin (ox + 1, oy + 1) (footprint origin (ox, oy)), nbx x nby tiles of
+// NV values each.
+template <class R, int NV>
+__device__ __forceinline__ void fp_sum(const R* __restrict__ fp, int x, int y, int ox, int oy, int nbx, int nby,
+                                       int v0, int nv, R* out) {
+  int bys[2], bxs[2];
+  const int ny = fp_tiles(x, ox, ABY, FPX, nby, bys), nx = fp_tiles(y, oy, ABX, FPY, nbx, bxs);
+  for (int v = 0; v < nv; v++) out[v] = R(0);
+  for (int i = 0; i < ny; i++)
+    for (int j = 0; j < nx; j++) {
+      const size_t tile = (size_t)bys[i] * nbx + bxs[j];
+      const int f = (x - (ox + ABY * bys[i])) * FPY + (y - (oy + ABX * bxs[j]));
+      for (int v = 0; v < nv; v++) out[v] += fp[(tile * NV + v0 + v) * FPN + f];
+    }
+}
+
+// five-point direction of stencil offset (i-1, j-1), -1 if none
+__host__ __device__ constexpr int d5_of(int i, int j) {
+  return (i == 1 && j == 1) ? 0 : (i == 2 && j == 1) ? 1 : (i == 0 && j == 1) ? 2 : (i == 1 && j == 2) ? 3
+                                                                                 : (i == 1 && j == 0) ? 4 : -1;
+}
 
 __device__ __forceinline__ void block_sum4(double* v, double* red) {
   const int t = threadIdx.y * blockDim.x + threadIdx.x;
@@ -45,22 +79,26 @@ __device__ __forceinline__ void load_v(const DevParams<R>& p, const R* phi, int 
       for (int j = 0; j < 3; j++) v[a][i][j] = phi[idx(p, a, x + i - 1, y + j - 1)];
 }
 
-// reverse of the mu/phi update, one thread per interior cell
+// reverse of the mu/phi update, one thread per interior cell; partials
+// gathered over the tile's footprint (phi_0, phi_1, mu, delta-phi_0, delta-phi_1)
 template <class R>
 __global__ void __launch_bounds__(256) GP_HEAVY k_adj_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                   const R* __restrict__ mu, const R* __restrict__ dphi,
                                                   const R* __restrict__ lam_phi, const R* __restrict__ lam_mu,
-                                                  R* __restrict__ cg, double* __restrict__ part) {
+                                                  R* __restrict__ fpc, double* __restrict__ part) {
   __shared__ double red[ADJ_NPAR * 256];
+  __shared__ R acc[FC_N * FPN];
   const DevParams<R>& p = *prm;
-  const int y = 3 + blockIdx.x * ABX + threadIdx.x;
-  const int x = 3 + blockIdx.y * ABY + threadIdx.y;
+  const int tx = threadIdx.x, ty = threadIdx.y, t = ty * ABX + tx;
+  const int y = 3 + blockIdx.x * ABX + tx;
+  const int x = 3 + blockIdx.y * ABY + ty;
+  for (int i = t; i < FC_N * FPN; i += 256) acc[i] = R(0);
   double par[ADJ_NPAR] = {0, 0, 0, 0};
+  R vb[2][3][3] = {}, mub5[5] = {}, dpb5[5][2] = {};
   if (x <= p.mesh_x + 2 && y <= p.mesh_y + 2) {
     const size_t o = (size_t)x * p.pitch + y;
     R seedp[2] = {lam_phi[o], lam_phi[p.plane + o]};
     R seedm = lam_mu[o];
-    R vb[2][3][3] = {}, mub5[5] = {}, dpb5[5][2] = {};
     if (seedp[0] != R(0) || seedp[1] != R(0) || seedm != R(0)) {
       R v[2][3][3], mu5[5], dp5[5][2];
       load_v(p, phi, x, y, v);
@@ -70,93 +108,109 @@ __global__ void __launch_bounds__(256) GP_HEAVY k_adj_conc(const DevParams<R>* _
       }
       adj_conc_cell(p, v, mu5, dp5, seedp, seedm, y, vb, mub5, dpb5, par);
     }
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++)
-        for (int a = 0; a < 2; a++) cg[(size_t)(CG_PHI + (i * 3 + j) * 2 + a) * p.plane + o] = vb[a][i][j];
-    for (int d = 0; d < 5; d++) {
-      cg[(size_t)(CG_MU + d) * p.plane + o] = mub5[d];
-      for (int a = 0; a < 2; a++) cg[(size_t)(CG_DP + d * 2 + a) * p.plane + o] = dpb5[d][a];
-    }
   }
+  __syncthreads();
+  // target (x+i-1, y+j-1) is footprint cell (ty+i, tx+j)
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int f = (ty + i) * FPY + (tx + j);
+      acc[f] += vb[0][i][j];
+      acc[FPN + f] += vb[1][i][j];
+      constexpr int dd[3][3] = {{d5_of(0, 0), d5_of(0, 1), d5_of(0, 2)},
+                                {d5_of(1, 0), d5_of(1, 1), d5_of(1, 2)},
+                                {d5_of(2, 0), d5_of(2, 1), d5_of(2, 2)}};
+      const int d = dd[i][j];
+      if (d >= 0) {
+        acc[2 * FPN + f] += mub5[d];
+        acc[3 * FPN + f] += dpb5[d][0];
+        acc[4 * FPN + f] += dpb5[d][1];
+      }
+      __syncthreads();
+    }
+  R* out = fpc + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * FC_N * FPN;
+  for (int i = t; i < FC_N * FPN; i += 256) out[i] = acc[i];
   block_sum4(par, red);
-  if (threadIdx.x == 0 && threadIdx.y == 0)
+  if (t == 0)
     for (int k = 0; k < ADJ_NPAR; k++) part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * ADJ_NPAR + k] = par[k];
 }
 
 __device__ __forceinline__ bool in_interior(int x, int y, int mx, int my) { return x >= 3 && x <= mx + 2 && y >= 3 && y <= my + 2; }
 
-// adjoint of delta-phi gathered from the update stencils, then the reverse of
-// the phase sweep, one thread per cell of [2, rows-3]^2
+// adjoint of delta-phi gathered from the update footprints, then the reverse
+// of the phase sweep, one thread per cell of [2, rows-3]^2; partials gathered
+// over the tile's footprint (phi_0, phi_1, mu).  nbx_c x nby_c: update tiles.
 template <class R>
 __global__ void __launch_bounds__(256) GP_HEAVY k_adj_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
-                                                   const R* __restrict__ mu, const R* __restrict__ cg,
-                                                   R* __restrict__ cf, double* __restrict__ part) {
+                                                   const R* __restrict__ mu, const R* __restrict__ fpc, int nbx_c,
+                                                   int nby_c, R* __restrict__ fpp, double* __restrict__ part) {
   __shared__ double red[ADJ_NPAR * 256];
+  __shared__ R acc[FF_N * FPN];
   const DevParams<R>& p = *prm;
-  const int y = 2 + blockIdx.x * ABX + threadIdx.x;
-  const int x = 2 + blockIdx.y * ABY + threadIdx.y;
+  const int tx = threadIdx.x, ty = threadIdx.y, t = ty * ABX + tx;
+  const int y = 2 + blockIdx.x * ABX + tx;
+  const int x = 2 + blockIdx.y * ABY + ty;
+  for (int i = t; i < FF_N * FPN; i += 256) acc[i] = R(0);
   double par[ADJ_NPAR] = {0, 0, 0, 0};
+  R vb[2][3][3] = {}, mub = R(0);
   if (x <= p.rx - 3 && y <= p.ry - 3) {
-    const size_t o = (size_t)x * p.pitch + y;
-    R seed[2] = {R(0), R(0)};
-    for (int d = 0; d < 5; d++) {
-      const int cx = x - D5X[d], cy = y - D5Y[d];
-      if (!in_interior(cx, cy, p.mesh_x, p.mesh_y)) continue;
-      const size_t oc = (size_t)cx * p.pitch + cy;
-      for (int a = 0; a < 2; a++) seed[a] += cg[(size_t)(CG_DP + d * 2 + a) * p.plane + oc];
-    }
-    R vb[2][3][3] = {}, mub = R(0);
+    R seed[2];
+    fp_sum<R, FC_N>(fpc, x, y, 2, 2, nbx_c, nby_c, 3, 2, seed);
     if (seed[0] != R(0) || seed[1] != R(0)) {
       R v[2][3][3];
       load_v(p, phi, x, y, v);
-      adj_phase_cell(p, v, mu[o], seed, vb, mub, par);
+      adj_phase_cell(p, v, mu[(size_t)x * p.pitch + y], seed, vb, mub, par);
     }
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++)
-        for (int a = 0; a < 2; a++) cf[(size_t)(CF_PHI + (i * 3 + j) * 2 + a) * p.plane + o] = vb[a][i][j];
-    cf[(size_t)CF_MU * p.plane + o] = mub;
   }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int f = (ty + i) * FPY + (tx + j);
+      acc[f] += vb[0][i][j];
+      acc[FPN + f] += vb[1][i][j];
+      if (i == 1 && j == 1) acc[2 * FPN + f] += mub;
+      __syncthreads();
+    }
+  R* out = fpp + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * FF_N * FPN;
+  for (int i = t; i < FF_N * FPN; i += 256) out[i] = acc[i];
   block_sum4(par, red);
-  if (threadIdx.x == 0 && threadIdx.y == 0)
+  if (t == 0)
     for (int k = 0; k < ADJ_NPAR; k++) part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * ADJ_NPAR + k] = par[k];
 }
 
-// Sum of the stencil partials pointing at cell (x, y) of [1, rows-2]^2:
-// sp[0..1] for phi, sm for mu.
+// Sum of the partials pointing at cell (x, y) of [1, rows-2]^2: sp[0..1] for
+// phi, sm for mu (update footprints, then phase footprints).
+struct FpGrid {
+  int nbx_c, nby_c, nbx_f, nby_f;  // update tiles, phase tiles
+};
+
 template <class R>
-__device__ __forceinline__ void gather_at(const DevParams<R>& p, const R* __restrict__ cg, const R* __restrict__ cf,
-                                          int x, int y, R* sp, R& sm) {
-  sp[0] = sp[1] = sm = R(0);
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) {
-      const int cx = x - (i - 1), cy = y - (j - 1);
-      const size_t oc = (size_t)cx * p.pitch + cy;
-      const int dir = i * 3 + j;
-      if (in_interior(cx, cy, p.mesh_x, p.mesh_y))
-        for (int a = 0; a < 2; a++) sp[a] += cg[(size_t)(CG_PHI + dir * 2 + a) * p.plane + oc];
-      if (cx >= 2 && cx <= p.rx - 3 && cy >= 2 && cy <= p.ry - 3)
-        for (int a = 0; a < 2; a++) sp[a] += cf[(size_t)(CF_PHI + dir * 2 + a) * p.plane + oc];
-    }
-  for (int d = 0; d < 5; d++) {
-    const int cx = x - D5X[d], cy = y - D5Y[d];
-    if (in_interior(cx, cy, p.mesh_x, p.mesh_y)) sm += cg[(size_t)(CG_MU + d) * p.plane + (size_t)cx * p.pitch + cy];
-  }
-  if (x >= 2 && x <= p.rx - 3 && y >= 2 && y <= p.ry - 3) sm += cf[(size_t)CF_MU * p.plane + (size_t)x * p.pitch + y];
+__device__ __forceinline__ void gather_at(const R* __restrict__ fpc, const R* __restrict__ fpp, const FpGrid& g, int x,
+                                          int y, R* sp, R& sm) {
+  R a[3], b[3];
+  fp_sum<R, FC_N>(fpc, x, y, 2, 2, g.nbx_c, g.nby_c, 0, 3, a);
+  fp_sum<R, FF_N>(fpp, x, y, 1, 1, g.nbx_f, g.nby_f, 0, 3, b);
+  sp[0] = a[0] + b[0];
+  sp[1] = a[1] + b[1];
+  sm = a[2] + b[2];
 }
 
 // New adjoint of every interior cell: the partials pointing at it plus those
 // pointing at its ghost images (the reverse of the ghost copies, which write
 // an interior value to the cells write_with_ghosts names).
 template <class R>
-__global__ void __launch_bounds__(256) k_adj_gather(const DevParams<R>* __restrict__ prm, const R* __restrict__ cg,
-                                                    const R* __restrict__ cf, R* __restrict__ lam_phi,
+__global__ void __launch_bounds__(256) k_adj_gather(const DevParams<R>* __restrict__ prm, const R* __restrict__ fpc,
+                                                    const R* __restrict__ fpp, FpGrid g, R* __restrict__ lam_phi,
                                                     R* __restrict__ lam_mu) {
   const DevParams<R>& p = *prm;
   const int y = 3 + blockIdx.x * ABX + threadIdx.x;
   const int x = 3 + blockIdx.y * ABY + threadIdx.y;
   if (x > p.mesh_x + 2 || y > p.mesh_y + 2) return;
   R sp[2], sm;
-  gather_at(p, cg, cf, x, y, sp, sm);
+  gather_at(fpc, fpp, g, x, y, sp, sm);
   // images: ghost rows/columns whose copy source is x / y, per field
   for (int f = 0; f < 2; f++) {
     int tx[3], ty[3], nx = 1, ny = 1;
@@ -172,7 +226,7 @@ __global__ void __launch_bounds__(256) k_adj_gather(const DevParams<R>* __restri
         const int gx = tx[i], gy = ty[j];
         if (gx < 1 || gx > p.rx - 2 || gy < 1 || gy > p.ry - 2) continue;  // never read by a stencil
         R gp[2], gm;
-        gather_at(p, cg, cf, gx, gy, gp, gm);
+        gather_at(fpc, fpp, g, gx, gy, gp, gm);
         if (f == 0) {
           sp[0] += gp[0];
           sp[1] += gp[1];
@@ -215,7 +269,7 @@ struct gp_adjoint {
   bool direct;           // one slot per step: the forward steps slot k into slot k+1 and keeps delta-phi
   size_t dphi_off;       // byte offset of delta-phi in a slot (direct)
   std::vector<void *> slots;
-  void *cg, *cf;         // partial planes of the update and phase reverses
+  void *fpc, *fpp;       // footprint sums of the update and phase reverse tiles
   void *lam[2][2];       // [buffer][0 = phi (2 planes), 1 = mu]
   int lcur;
   double *part, *acc;
@@ -353,12 +407,13 @@ static int adj_step(gp_adjoint *A, const void *phi_s, const void *mu_s, const vo
   dim3 g_ring((c->ry - 4 + ABX - 1) / ABX, (c->rx - 4 + ABY - 1) / ABY);
   const int li = A->lcur, lo = A->lcur ^ 1;
   const int nb_in = (int)(g_in.x * g_in.y), nb_ring = (int)(g_ring.x * g_ring.y);
+  const FpGrid fg = {(int)g_in.x, (int)g_in.y, (int)g_ring.x, (int)g_ring.y};
   hipLaunchKernelGGL((k_adj_conc<R>), g_in, ab, 0, c->stream, prm, phi, mu, dphi,
-                     (const R *)A->lam[li][0], (const R *)A->lam[li][1], (R *)A->cg, A->part);
-  hipLaunchKernelGGL((k_adj_phase<R>), g_ring, ab, 0, c->stream, prm, phi, mu, (const R *)A->cg, (R *)A->cf,
-                     A->part + (size_t)nb_in * ADJ_NPAR);
+                     (const R *)A->lam[li][0], (const R *)A->lam[li][1], (R *)A->fpc, A->part);
+  hipLaunchKernelGGL((k_adj_phase<R>), g_ring, ab, 0, c->stream, prm, phi, mu, (const R *)A->fpc, fg.nbx_c, fg.nby_c,
+                     (R *)A->fpp, A->part + (size_t)nb_in * ADJ_NPAR);
   hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(256), 0, c->stream, (const double *)A->part, nb_in + nb_ring, A->acc);
-  hipLaunchKernelGGL((k_adj_gather<R>), g_in, ab, 0, c->stream, prm, (const R *)A->cg, (const R *)A->cf,
+  hipLaunchKernelGGL((k_adj_gather<R>), g_in, ab, 0, c->stream, prm, (const R *)A->fpc, (const R *)A->fpp, fg,
                      (R *)A->lam[lo][0], (R *)A->lam[lo][1]);
   GP_HIP_CHECK(hipGetLastError());
   GP_HIP_CHECK(hipEventRecord(ep[1], c->stream));
@@ -499,7 +554,13 @@ int gp_adjoint_create(gp_ctx *c, long nsteps, int nsnap, gp_adjoint **out) {
   bool ok = true;
   A->slots.assign(A->nslots, nullptr);
   for (int i = 0; i < A->nslots && ok; i++) ok = hipMalloc(&A->slots[i], A->slot_bytes) == hipSuccess;
-  ok = ok && hipMalloc(&A->cg, pb * gpk::CG_N) == hipSuccess && hipMalloc(&A->cf, pb * gpk::CF_N) == hipSuccess;
+  {
+    const long MX = c->hp.mesh_x, MY = c->hp.mesh_y;
+    const size_t nb_in = (size_t)((MY + gpk::ABX - 1) / gpk::ABX) * ((MX + gpk::ABY - 1) / gpk::ABY);
+    const size_t nb_ring = (size_t)((c->ry - 4 + gpk::ABX - 1) / gpk::ABX) * ((c->rx - 4 + gpk::ABY - 1) / gpk::ABY);
+    ok = ok && hipMalloc(&A->fpc, nb_in * gpk::FC_N * gpk::FPN * c->esize) == hipSuccess &&
+         hipMalloc(&A->fpp, nb_ring * gpk::FF_N * gpk::FPN * c->esize) == hipSuccess;
+  }
   for (int b = 0; b < 2 && ok; b++)
     ok = hipMalloc(&A->lam[b][0], pb * 2) == hipSuccess && hipMalloc(&A->lam[b][1], pb) == hipSuccess &&
          hipMemset(A->lam[b][0], 0, pb * 2) == hipSuccess && hipMemset(A->lam[b][1], 0, pb) == hipSuccess;
@@ -521,7 +582,7 @@ void gp_adjoint_destroy(gp_adjoint *A) {
   if (A->c) (void)hipStreamSynchronize(A->c->stream);
   for (void *s : A->slots)
     if (s) (void)hipFree(s);
-  for (void *p : {A->cg, A->cf, A->lam[0][0], A->lam[0][1], A->lam[1][0], A->lam[1][1]})
+  for (void *p : {A->fpc, A->fpp, A->lam[0][0], A->lam[0][1], A->lam[1][0], A->lam[1][1]})
     if (p) (void)hipFree(p);
   if (A->part) (void)hipFree(A->part);
   if (A->acc) (void)hipFree(A->acc);
