@@ -181,6 +181,25 @@ __global__ void __launch_bounds__(256) GP_HEAVY k_adj_phase(const DevParams<R>* 
     for (int k = 0; k < ADJ_NPAR; k++) part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * ADJ_NPAR + k] = par[k];
 }
 
+// fixed-order sum of the block partials into the gradient accumulator, by one
+// block of 256 threads
+__device__ __forceinline__ void adj_reduce_block(const double* __restrict__ part, int nblk, double* __restrict__ acc,
+                                                 double* red) {
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  double v[ADJ_NPAR] = {0, 0, 0, 0};
+  for (int b = t; b < nblk; b += 256)
+    for (int k = 0; k < ADJ_NPAR; k++) v[k] += part[(size_t)b * ADJ_NPAR + k];
+  for (int k = 0; k < ADJ_NPAR; k++) red[k * 256 + t] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+      for (int k = 0; k < ADJ_NPAR; k++) red[k * 256 + t] += red[k * 256 + t + s];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int k = 0; k < ADJ_NPAR; k++) acc[k] += red[k * 256];
+}
+
 // Sum of the partials pointing at cell (x, y) of [1, rows-2]^2: sp[0..1] for
 // phi, sm for mu (update footprints, then phase footprints).
 struct FpGrid {
@@ -201,10 +220,16 @@ __device__ __forceinline__ void gather_at(const R* __restrict__ fpc, const R* __
 // New adjoint of every interior cell: the partials pointing at it plus those
 // pointing at its ghost images (the reverse of the ghost copies, which write
 // an interior value to the cells write_with_ghosts names).
+// The first block also sums the parameter partials of this step's update and
+// phase reverses into acc (they are complete when the kernel starts; the work
+// overlaps the other blocks' gathers).
 template <class R>
 __global__ void __launch_bounds__(256) k_adj_gather(const DevParams<R>* __restrict__ prm, const R* __restrict__ fpc,
                                                     const R* __restrict__ fpp, FpGrid g, R* __restrict__ lam_phi,
-                                                    R* __restrict__ lam_mu) {
+                                                    R* __restrict__ lam_mu, const double* __restrict__ part, int nblk,
+                                                    double* __restrict__ acc) {
+  __shared__ double red[ADJ_NPAR * 256];
+  if (blockIdx.x == 0 && blockIdx.y == 0) adj_reduce_block(part, nblk, acc, red);
   const DevParams<R>& p = *prm;
   const int y = 3 + blockIdx.x * ABX + threadIdx.x;
   const int x = 3 + blockIdx.y * ABY + threadIdx.y;
@@ -239,24 +264,6 @@ __global__ void __launch_bounds__(256) k_adj_gather(const DevParams<R>* __restri
   lam_phi[o] = sp[0];
   lam_phi[p.plane + o] = sp[1];
   lam_mu[o] = sm;
-}
-
-// fixed-order sum of the block partials into the gradient accumulator
-__global__ void k_adj_reduce(const double* __restrict__ part, int nblk, double* __restrict__ acc) {
-  __shared__ double red[ADJ_NPAR * 256];
-  const int t = threadIdx.x;
-  double v[ADJ_NPAR] = {0, 0, 0, 0};
-  for (int b = t; b < nblk; b += 256)
-    for (int k = 0; k < ADJ_NPAR; k++) v[k] += part[(size_t)b * ADJ_NPAR + k];
-  for (int k = 0; k < ADJ_NPAR; k++) red[k * 256 + t] = v[k];
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s)
-      for (int k = 0; k < ADJ_NPAR; k++) red[k * 256 + t] += red[k * 256 + t + s];
-    __syncthreads();
-  }
-  if (t == 0)
-    for (int k = 0; k < ADJ_NPAR; k++) acc[k] += red[k * 256];
 }
 
 }  // namespace gpk
@@ -412,9 +419,8 @@ static int adj_step(gp_adjoint *A, const void *phi_s, const void *mu_s, const vo
                      (const R *)A->lam[li][0], (const R *)A->lam[li][1], (R *)A->fpc, A->part);
   hipLaunchKernelGGL((k_adj_phase<R>), g_ring, ab, 0, c->stream, prm, phi, mu, (const R *)A->fpc, fg.nbx_c, fg.nby_c,
                      (R *)A->fpp, A->part + (size_t)nb_in * ADJ_NPAR);
-  hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(256), 0, c->stream, (const double *)A->part, nb_in + nb_ring, A->acc);
   hipLaunchKernelGGL((k_adj_gather<R>), g_in, ab, 0, c->stream, prm, (const R *)A->fpc, (const R *)A->fpp, fg,
-                     (R *)A->lam[lo][0], (R *)A->lam[lo][1]);
+                     (R *)A->lam[lo][0], (R *)A->lam[lo][1], (const double *)A->part, nb_in + nb_ring, A->acc);
   GP_HIP_CHECK(hipGetLastError());
   GP_HIP_CHECK(hipEventRecord(ep[1], c->stream));
   A->lcur = lo;
