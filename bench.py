@@ -276,6 +276,10 @@ def main():
         a.steps = int(min(nsteps_full, max(1, 2e10 // (MX * MY))))
     s = gpstep.Solver(inp, a.dtype, path=a.path, graphs=not a.no_graphs)
     s.set_state(phi0, mu0)
+    # the automatic path times one graph block of each candidate path: a warm-up
+    # of at least three blocks captures all their graphs outside the timed region
+    if a.path < 0 and not a.no_graphs:
+        a.warmup = max(a.warmup, 3 * 32)
     s.step(a.warmup)
     s.set_state(phi0, mu0)  # the timed steps start from the initial state
     if a.path < 0:
@@ -289,7 +293,8 @@ def main():
     names = ["sparse: fused LDS kernel on interface tiles + bulk kernel", "two-pass", "dense fused",
              "sparse: two-pass kernels on interface tiles + bulk kernel"]
     path_last, auto = s.path()
-    path_desc = (f"automatic (one timed block of each of two-pass, sparse fused, sparse two-pass every 2048 steps; "
+    path_desc = (f"automatic (one timed 32-step block of each of two-pass, sparse fused, sparse two-pass at the start, "
+                 f"after 64 blocks, then at gaps doubling up to 512 blocks while the choice holds; "
                  f"last: {names[path_last]})" if auto else names[path_last])
     sched = s.sched_info() if s.path()[0] != 1 else None
     st = s.stats(allow_nonfinite=True)

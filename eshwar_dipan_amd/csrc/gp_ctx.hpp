@@ -48,9 +48,10 @@ struct gp_ctx {
   // captured step blocks per (path, tile shape) and starting parity
   hipGraphExec_t gcache[GP_GRAPH_SLOTS][2];
   int gkey[GP_GRAPH_SLOTS];
-  // automatic path choice: re-timed every auto_period blocks
+  // automatic path choice: re-timed when auto_count reaches auto_next; the gap
+  // starts at auto_period blocks and doubles (up to 8x) while the choice holds
   int auto_mode;
-  long auto_count, auto_period;
+  long auto_count, auto_period, auto_next, auto_gap;
   // grids of the tile set: a context steps itself; an ensemble master (nmember
   // > 0) owns no fields and steps its member contexts as one tile set
   int nmember;

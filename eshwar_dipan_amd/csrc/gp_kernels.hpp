@@ -4,12 +4,13 @@
 // `pitch` elements (y fastest, pitch >= MESH_Y+6 rounded to 256 bytes), i.e. the
 // reference's padded grid with its three ghost layers (solverloop/serialinfo_xy.h:7-16).
 //
-// Two-pass path (parity reference kernels):
-//   k_phase : delta-phi on x,y in [2, rows-3] (interior plus the ghost ring the
-//             antitrapping fluxes read)        -- solverloop_phasefield
-//   k_conc  : mu and phi update of the interior, written out-of-place, each cell
-//             also written to the ghost cells that copy it (so the ghost copies of
-//             apply_boundary_conditions need no separate pass) -- solverloop_concentration
+// Two-pass path:
+//   k_phase      : delta-phi on x,y in [2, rows-3] (interior plus the ghost ring
+//                  the antitrapping fluxes read)        -- solverloop_phasefield
+//   k_conc_lds / : mu and phi update of the interior, written out-of-place, each
+//   k_conc_march   cell also written to the ghost cells that copy it (so the ghost
+//                  copies of apply_boundary_conditions need no separate pass);
+//                  antitrapping faces shared between cells -- solverloop_concentration
 // Fused path (one launch per step): see k_fused in gp_fused.hpp.
 #pragma once
 
@@ -222,19 +223,7 @@ __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_phase(const DevParams<R>* 
   phase_cell<P, C>(p, phi, mu, dphi, x, y);
 }
 
-template <int P, int C, class R>
-__global__ void __launch_bounds__(BX * BY) GP_HEAVY k_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
-                                                  const R* __restrict__ mu, const R* __restrict__ dphi,
-                                                  const R* __restrict__ mulo, R* __restrict__ phi_o,
-                                                  R* __restrict__ mu_o, R* __restrict__ mulo_o) {
-  const DevParams<R>& p = *prm;
-  const int y = 3 + blockIdx.x * BX + threadIdx.x;
-  const int x = 3 + blockIdx.y * BY + threadIdx.y;
-  if (x > p.mesh_x + 2 || y > p.mesh_y + 2) return;
-  conc_cell<P, C, R, !Ops<R>::Exact>(p, phi, mu, dphi, mulo, phi_o, mu_o, mulo_o, x, y);
-}
-
-// Face-sharing variant of k_conc.  The antitrapping flux of a face is the same
+// mu and phi update of the interior with shared faces.  The antitrapping flux of a face is the same
 // function of the same operands whichever of its two cells evaluates it
 // (face_jat on the pair (lower, upper)), so a face need not be evaluated
 // twice: a wave holds 64 consecutive y columns, starting one column before the
@@ -408,22 +397,25 @@ __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_conc_lds(const DevParams<R
 }
 
 // One two-pass step on `stream`: delta-phi of (phi, mu) into dphi, then the
-// update into (phi_o, mu_o[, mulo_o]).  Binary64 shares x faces through LDS
-// (3% faster at 1024^2, P=4); in binary32 the barrier costs more than the face
-// it saves (1% slower), so one row per wave.
+// update into (phi_o, mu_o[, mulo_o]) with shared antitrapping faces.
+// Binary64 shares x faces through LDS (3% faster at 1024^2, P=4); in binary32
+// the barrier costs more than the face it saves (1% slower), so one row per
+// wave.  The sharing kernels beat one cell per thread also at two phases
+// (512^2: 1.3% in binary64, 5% in binary32; equal at 256^2).
 template <int P, int C, class R>
 void launch_two_pass(const DevParams<R>* prm, int rx, int ry, int mesh_x, int mesh_y, hipStream_t stream,
                      const R* phi, const R* mu, R* dphi, const R* mulo, R* phi_o, R* mu_o, R* mulo_o) {
+  const dim3 blk(BX, BY);
+  hipLaunchKernelGGL((k_phase<P, C, R>), phase_grid(rx, ry), blk, 0, stream, prm, phi, mu, dphi);
   constexpr bool lds = Ops<R>::Exact;
   constexpr int rows = BY * (lds ? 1 : KM);
-  const dim3 blk(BX, BY), g_conc((mesh_y + MW - 1) / MW, (mesh_x + rows - 1) / rows);
-  hipLaunchKernelGGL((k_phase<P, C, R>), phase_grid(rx, ry), blk, 0, stream, prm, phi, mu, dphi);
+  const dim3 g((mesh_y + MW - 1) / MW, (mesh_x + rows - 1) / rows);
   if constexpr (lds)
-    hipLaunchKernelGGL((k_conc_lds<P, C, R>), g_conc, blk, 0, stream, prm, phi, mu, (const R*)dphi, mulo, phi_o,
-                       mu_o, mulo_o);
+    hipLaunchKernelGGL((k_conc_lds<P, C, R>), g, blk, 0, stream, prm, phi, mu, (const R*)dphi, mulo, phi_o, mu_o,
+                       mulo_o);
   else
-    hipLaunchKernelGGL((k_conc_march<P, C, R>), g_conc, blk, 0, stream, prm, phi, mu, (const R*)dphi, mulo, phi_o,
-                       mu_o, mulo_o);
+    hipLaunchKernelGGL((k_conc_march<P, C, R>), g, blk, 0, stream, prm, phi, mu, (const R*)dphi, mulo, phi_o, mu_o,
+                       mulo_o);
 }
 
 // Fill the ghost layers of a freshly loaded state: every interior cell copies

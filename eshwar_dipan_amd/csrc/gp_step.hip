@@ -283,6 +283,8 @@ int gp_set_path(gp_ctx *c, int path) {
   if (!c || path < -1 || path > 3 || (c->nmember && path != 3 && path != -1)) return GP_E_ARG;
   c->auto_mode = path < 0;
   c->auto_count = 0;
+  c->auto_next = 0;
+  c->auto_gap = c->auto_period;
   c->path = path < 0 ? (c->nmember ? 3 : 1) : path;
   return GP_OK;
 }
@@ -444,9 +446,13 @@ int gp_step_impl(gp_ctx *c, long nsteps, bool timed) {
   // graphs: on a capturable stream, for runs of at least one block
   if (c->use_graphs && c->stream && left >= GP_GRAPH_STEPS) {
     while (!rc && left >= GP_GRAPH_STEPS) {
-      if (c->auto_mode && c->auto_count % c->auto_period == 0 && left >= 3 * GP_GRAPH_STEPS) {
+      if (c->auto_mode && c->auto_count >= c->auto_next && left >= 3 * GP_GRAPH_STEPS) {
+        const int before = c->path;
+        const bool first = c->auto_count == 0;
         rc = retune(c, &left);
         c->auto_count++;
+        c->auto_gap = (!first && c->path == before) ? std::min(2 * c->auto_gap, 8 * c->auto_period) : c->auto_period;
+        c->auto_next = c->auto_count + c->auto_gap;
         continue;
       }
       c->auto_count++;
