@@ -393,15 +393,17 @@ static int adj_seed(gp_adjoint *A) {
 // one reverse step from the state S_k at (phi, mu) (the context's buffers or
 // a snapshot slot, same padded layout): lam <- J_k^T lam
 template <class R>
-static int adj_step(gp_adjoint *A, const void *phi_s, const void *mu_s, const void *dphi_s) {
+static int adj_step(gp_adjoint *A, const void *phi_s, const void *mu_s, const void *dphi_s, bool timed = true) {
   gp_ctx *c = A->c;
   using namespace gpk;
   const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
   const R *phi = (const R *)phi_s, *mu = (const R *)mu_s;
-  hipEvent_t *ep;
-  int rc = ev_pair(A, 1, &ep);
-  if (rc) return rc;
-  GP_HIP_CHECK(hipEventRecord(ep[0], c->stream));
+  hipEvent_t *ep = nullptr;
+  if (timed) {
+    int rc = ev_pair(A, 1, &ep);
+    if (rc) return rc;
+    GP_HIP_CHECK(hipEventRecord(ep[0], c->stream));
+  }
   const R *dphi = (const R *)dphi_s;
   if (!dphi) {  // delta-phi of S_k not kept: recompute it
     hipLaunchKernelGGL((k_phase<2, 1, R>), phase_grid(c->rx, c->ry), dim3(BX, BY), 0, c->stream, prm, phi, mu,
@@ -422,7 +424,7 @@ static int adj_step(gp_adjoint *A, const void *phi_s, const void *mu_s, const vo
   hipLaunchKernelGGL((k_adj_gather<R>), g_in, ab, 0, c->stream, prm, (const R *)A->fpc, (const R *)A->fpp, fg,
                      (R *)A->lam[lo][0], (R *)A->lam[lo][1], (const double *)A->part, nb_in + nb_ring, A->acc);
   GP_HIP_CHECK(hipGetLastError());
-  GP_HIP_CHECK(hipEventRecord(ep[1], c->stream));
+  if (timed) GP_HIP_CHECK(hipEventRecord(ep[1], c->stream));
   A->lcur = lo;
   return GP_OK;
 }
@@ -474,10 +476,14 @@ static int adj_run_direct(gp_adjoint *A) {
   if ((rc = adj_forward_direct<R>(A))) return rc;
   if ((rc = adj_restore(A, (int)A->nsteps))) return rc;  // the context holds S_n for the seed
   if ((rc = adj_seed<R>(A))) return rc;
+  hipEvent_t *e;
+  if ((rc = ev_pair(A, 1, &e))) return rc;
+  GP_HIP_CHECK(hipEventRecord(e[0], c->stream));
   for (long k = A->nsteps - 1; k >= 0; k--) {
     const char *sk = (const char *)A->slots[k];
-    if ((rc = adj_step<R>(A, sk, sk + pb * c->P, sk + A->dphi_off))) return rc;
+    if ((rc = adj_step<R>(A, sk, sk + pb * c->P, sk + A->dphi_off, false))) return rc;
   }
+  GP_HIP_CHECK(hipEventRecord(e[1], c->stream));
   return GP_OK;
 }
 
@@ -575,6 +581,14 @@ int gp_adjoint_create(gp_ctx *c, long nsteps, int nsnap, gp_adjoint **out) {
   ok = ok && hipMalloc((void **)&A->part, sizeof(double) * gpk::ADJ_NPAR * A->nblk) == hipSuccess &&
        hipMalloc((void **)&A->acc, sizeof(double) * gpk::ADJ_NPAR) == hipSuccess;
   ok = ok && hipEventCreate(&A->e0) == hipSuccess && hipEventCreate(&A->e1) == hipSuccess;
+  // timing events, created here rather than during a run: two pairs in direct
+  // mode, about one pair per reverse step and per recomputation otherwise
+  const long npairs = A->direct ? 2 : 3 * nsteps + 8;
+  for (long i = 0; i < 2 * npairs && ok; i++) {
+    hipEvent_t e;
+    ok = hipEventCreate(&e) == hipSuccess;
+    if (ok) A->ev.push_back(e);
+  }
   if (!ok) {
     gp_adjoint_destroy(A);
     return GP_E_NOMEM;

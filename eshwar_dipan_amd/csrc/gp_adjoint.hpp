@@ -450,27 +450,22 @@ __device__ __forceinline__ void adj_phase_cell(const DevParams<R>& p, const R (&
         R uB[2] = {A.sXb * B.gXb - B.sXb * A.gXb, R(0.5) * (A.sXb * (B.cYb + B.cYc) - B.sXb * (A.cYb + A.cYc))};
         R uR[2] = {R(0.5) * (A.sYc * (B.cXc + B.cXr) - B.sYc * (A.cXc + A.cXr)), A.sYc * B.gYc - B.sYc * A.gYc};
         R uL[2] = {R(0.5) * (A.sYl * (B.cXl + B.cXc) - B.sYl * (A.cXl + A.cXc)), A.sYl * B.gYl - B.sYl * A.gYl};
-        R d0[2] = {R(0), R(0)};
-        R fz = R(0);
-        double dz = 0;
-        const R EF = face_E_rev(Mrot(a, b), Miv(a, b), dab, uF, uF[0], 0, R(0), d0, fz, dz);
-        const R EB = face_E_rev(Mrot(a, b), Miv(a, b), dab, uB, uB[0], 0, R(0), d0, fz, dz);
-        const R ER = face_E_rev(Mrot(a, b), Miv(a, b), dab, uR, uR[1], 1, R(0), d0, fz, dz);
-        const R EL = face_E_rev(Mrot(a, b), Miv(a, b), dab, uL, uL[1], 1, R(0), d0, fz, dz);
-        const R S = ((-B.sXc) * EF - (-B.sXb) * EB) * p.inv_dx + ((-B.sYc) * ER - (-B.sYl) * EL) * p.inv_dy;
-        gb += (double)(divb * R(2) * S);
+        // the seeds of the face terms do not depend on their values, so one
+        // evaluation per face gives both the value and its adjoint
         const R Sb = divb * R(2) * gam;
         const R GF = Sb * p.inv_dx, GB = -Sb * p.inv_dx, GR = Sb * p.inv_dy, GL = -Sb * p.inv_dy;
+        R uFb[2] = {R(0), R(0)}, uBb[2] = {R(0), R(0)}, uRb[2] = {R(0), R(0)}, uLb[2] = {R(0), R(0)};
+        R fF = R(0), fB = R(0), fR = R(0), fL = R(0);
+        const R EF = face_E_rev(Mrot(a, b), Miv(a, b), dab, uF, uF[0], 0, -B.sXc * GF, uFb, fF, db);
+        const R EB = face_E_rev(Mrot(a, b), Miv(a, b), dab, uB, uB[0], 0, -B.sXb * GB, uBb, fB, db);
+        const R ER = face_E_rev(Mrot(a, b), Miv(a, b), dab, uR, uR[1], 1, -B.sYc * GR, uRb, fR, db);
+        const R EL = face_E_rev(Mrot(a, b), Miv(a, b), dab, uL, uL[1], 1, -B.sYl * GL, uLb, fL, db);
+        const R S = ((-B.sXc) * EF - (-B.sXb) * EB) * p.inv_dx + ((-B.sYc) * ER - (-B.sYl) * EL) * p.inv_dy;
+        gb += (double)(divb * R(2) * S);
         Bb.sXc -= GF * EF;
         Bb.sXb -= GB * EB;
         Bb.sYc -= GR * ER;
         Bb.sYl -= GL * EL;
-        R uFb[2] = {R(0), R(0)}, uBb[2] = {R(0), R(0)}, uRb[2] = {R(0), R(0)}, uLb[2] = {R(0), R(0)};
-        R fF = R(0), fB = R(0), fR = R(0), fL = R(0);
-        face_E_rev(Mrot(a, b), Miv(a, b), dab, uF, uF[0], 0, -B.sXc * GF, uFb, fF, db);
-        face_E_rev(Mrot(a, b), Miv(a, b), dab, uB, uB[0], 0, -B.sXb * GB, uBb, fB, db);
-        face_E_rev(Mrot(a, b), Miv(a, b), dab, uR, uR[1], 1, -B.sYc * GR, uRb, fR, db);
-        face_E_rev(Mrot(a, b), Miv(a, b), dab, uL, uL[1], 1, -B.sYl * GL, uLb, fL, db);
         uFb[0] += fF;
         uBb[0] += fB;
         uRb[1] += fR;
