@@ -238,6 +238,84 @@ __device__ __forceinline__ R psi1(const DevParams<R>& p, R mu, int a, R& dpsi) {
 
 // ---------------------------------------------------------------- phase sweep
 
+// u vectors of the four faces of pair (A, B) (function_A_01.h:101-219)
+template <class R>
+__device__ __forceinline__ void face_us(const Stn<R>& A, const Stn<R>& B, R (&uF)[2], R (&uB)[2], R (&uR)[2],
+                                        R (&uL)[2]) {
+  uF[0] = A.sXc * B.gXc - B.sXc * A.gXc;
+  uF[1] = R(0.5) * (A.sXc * (B.cYc + B.cYf) - B.sXc * (A.cYc + A.cYf));
+  uB[0] = A.sXb * B.gXb - B.sXb * A.gXb;
+  uB[1] = R(0.5) * (A.sXb * (B.cYb + B.cYc) - B.sXb * (A.cYb + A.cYc));
+  uR[0] = R(0.5) * (A.sYc * (B.cXc + B.cXr) - B.sYc * (A.cXc + A.cXr));
+  uR[1] = A.sYc * B.gYc - B.sYc * A.gYc;
+  uL[0] = R(0.5) * (A.sYl * (B.cXl + B.cXc) - B.sYl * (A.cXl + A.cXc));
+  uL[1] = A.sYl * B.gYl - B.sYl * A.gYl;
+}
+
+// Face terms E (F, B, R, L) of pair (A, B) with seeds eb[4]: returns the values
+// in E and adds the adjoints to the stencil adjoints Ab, Bb and to dabb.
+template <class R>
+__device__ __forceinline__ void faces_rev(const R (&M)[2][2], const R (&Mi)[2][2], R dab, const Stn<R>& A,
+                                          const Stn<R>& B, Stn<R>& Ab, Stn<R>& Bb, const R* eb, R* E, double& dabb) {
+  R uF[2], uB[2], uR[2], uL[2];
+  face_us(A, B, uF, uB, uR, uL);
+  R uFb[2] = {R(0), R(0)}, uBb[2] = {R(0), R(0)}, uRb[2] = {R(0), R(0)}, uLb[2] = {R(0), R(0)};
+  R fF = R(0), fB = R(0), fR = R(0), fL = R(0);
+  E[0] = face_E_rev(M, Mi, dab, uF, uF[0], 0, eb[0], uFb, fF, dabb);
+  E[1] = face_E_rev(M, Mi, dab, uB, uB[0], 0, eb[1], uBb, fB, dabb);
+  E[2] = face_E_rev(M, Mi, dab, uR, uR[1], 1, eb[2], uRb, fR, dabb);
+  E[3] = face_E_rev(M, Mi, dab, uL, uL[1], 1, eb[3], uLb, fL, dabb);
+  uFb[0] += fF;
+  uBb[0] += fB;
+  uRb[1] += fR;
+  uLb[1] += fL;
+  // uF
+  Ab.sXc += uFb[0] * B.gXc;
+  Bb.gXc += uFb[0] * A.sXc;
+  Bb.sXc -= uFb[0] * A.gXc;
+  Ab.gXc -= uFb[0] * B.sXc;
+  Ab.sXc += uFb[1] * R(0.5) * (B.cYc + B.cYf);
+  Bb.cYc += uFb[1] * R(0.5) * A.sXc;
+  Bb.cYf += uFb[1] * R(0.5) * A.sXc;
+  Bb.sXc -= uFb[1] * R(0.5) * (A.cYc + A.cYf);
+  Ab.cYc -= uFb[1] * R(0.5) * B.sXc;
+  Ab.cYf -= uFb[1] * R(0.5) * B.sXc;
+  // uB
+  Ab.sXb += uBb[0] * B.gXb;
+  Bb.gXb += uBb[0] * A.sXb;
+  Bb.sXb -= uBb[0] * A.gXb;
+  Ab.gXb -= uBb[0] * B.sXb;
+  Ab.sXb += uBb[1] * R(0.5) * (B.cYb + B.cYc);
+  Bb.cYb += uBb[1] * R(0.5) * A.sXb;
+  Bb.cYc += uBb[1] * R(0.5) * A.sXb;
+  Bb.sXb -= uBb[1] * R(0.5) * (A.cYb + A.cYc);
+  Ab.cYb -= uBb[1] * R(0.5) * B.sXb;
+  Ab.cYc -= uBb[1] * R(0.5) * B.sXb;
+  // uR
+  Ab.sYc += uRb[0] * R(0.5) * (B.cXc + B.cXr);
+  Bb.cXc += uRb[0] * R(0.5) * A.sYc;
+  Bb.cXr += uRb[0] * R(0.5) * A.sYc;
+  Bb.sYc -= uRb[0] * R(0.5) * (A.cXc + A.cXr);
+  Ab.cXc -= uRb[0] * R(0.5) * B.sYc;
+  Ab.cXr -= uRb[0] * R(0.5) * B.sYc;
+  Ab.sYc += uRb[1] * B.gYc;
+  Bb.gYc += uRb[1] * A.sYc;
+  Bb.sYc -= uRb[1] * A.gYc;
+  Ab.gYc -= uRb[1] * B.sYc;
+  // uL
+  Ab.sYl += uLb[0] * R(0.5) * (B.cXl + B.cXc);
+  Bb.cXl += uLb[0] * R(0.5) * A.sYl;
+  Bb.cXc += uLb[0] * R(0.5) * A.sYl;
+  Bb.sYl -= uLb[0] * R(0.5) * (A.cXl + A.cXc);
+  Ab.cXl -= uLb[0] * R(0.5) * B.sYl;
+  Ab.cXc -= uLb[0] * R(0.5) * B.sYl;
+  Ab.sYl += uLb[1] * B.gYl;
+  Bb.gYl += uLb[1] * A.sYl;
+  Bb.sYl -= uLb[1] * A.gYl;
+  Ab.gYl -= uLb[1] * B.sYl;
+}
+
+
 // Reverse of delta-phi of one cell.  v: phi 3x3 blocks, mu: potential of the
 // cell, seed: adjoint of delta-phi.  Adds to vb (3x3 phi adjoints), mub, par.
 template <class R>
@@ -280,6 +358,25 @@ __device__ __forceinline__ void adj_phase_cell(const DevParams<R>& p, const R (&
   int nact = (int)act[0] + (int)act[1];
   // the forward lambda is needed for dp0 (projection branches); recompute it
   // with the same expressions as the reverse below
+  // With the pair data symmetric (rotations, dab), the faces of the ordered
+  // pair (1,0) see u negated: q and grad a_c change sign, a_c and |q|^2 do not,
+  // so each face term of phase 1 is minus that of phase 0.  The faces are then
+  // evaluated once for the pair, forward and reverse (with the two phases'
+  // seeds combined).
+  const bool shared = !an || p.pair_sym;
+  R E0[4] = {R(0), R(0), R(0), R(0)};  // face terms of the pair (0, 1) (shared)
+  auto face_vals = [&](int a, R* E) {
+    const int b = 1 - a;
+    const Stn<R>&A = s[a], &B = s[b];
+    R uF[2], uB[2], uR[2], uL[2], ub[2] = {R(0), R(0)}, fb = R(0);
+    double dd = 0;
+    face_us(A, B, uF, uB, uR, uL);
+    E[0] = face_E_rev(Mrot(a, b), Miv(a, b), dab, uF, uF[0], 0, R(0), ub, fb, dd);
+    E[1] = face_E_rev(Mrot(a, b), Miv(a, b), dab, uB, uB[0], 0, R(0), ub, fb, dd);
+    E[2] = face_E_rev(Mrot(a, b), Miv(a, b), dab, uR, uR[1], 1, R(0), ub, fb, dd);
+    E[3] = face_E_rev(Mrot(a, b), Miv(a, b), dab, uL, uL[1], 1, R(0), ub, fb, dd);
+  };
+  if (any && shared) face_vals(0, E0);
   auto lam_of = [&](int a) -> R {
     const int b = 1 - a;
     const Stn<R>&A = s[a], &B = s[b];
@@ -288,15 +385,14 @@ __device__ __forceinline__ void adj_phase_cell(const DevParams<R>& p, const R (&
     R ub[2] = {R(0), R(0)}, wb[2] = {R(0), R(0)};
     double dd = 0;
     R dA = R(2) * gam * dA_E2_rev(Mrot(a, b), dab, u, w, R(0), ub, wb, dd);
-    R fb = R(0);
-    R uF[2] = {A.sXc * B.gXc - B.sXc * A.gXc, R(0.5) * (A.sXc * (B.cYc + B.cYf) - B.sXc * (A.cYc + A.cYf))};
-    R uB[2] = {A.sXb * B.gXb - B.sXb * A.gXb, R(0.5) * (A.sXb * (B.cYb + B.cYc) - B.sXb * (A.cYb + A.cYc))};
-    R uR[2] = {R(0.5) * (A.sYc * (B.cXc + B.cXr) - B.sYc * (A.cXc + A.cXr)), A.sYc * B.gYc - B.sYc * A.gYc};
-    R uL[2] = {R(0.5) * (A.sYl * (B.cXl + B.cXc) - B.sYl * (A.cXl + A.cXc)), A.sYl * B.gYl - B.sYl * A.gYl};
-    R EF = face_E_rev(Mrot(a, b), Miv(a, b), dab, uF, uF[0], 0, R(0), ub, fb, dd);
-    R EB = face_E_rev(Mrot(a, b), Miv(a, b), dab, uB, uB[0], 0, R(0), ub, fb, dd);
-    R ER = face_E_rev(Mrot(a, b), Miv(a, b), dab, uR, uR[1], 1, R(0), ub, fb, dd);
-    R EL = face_E_rev(Mrot(a, b), Miv(a, b), dab, uL, uL[1], 1, R(0), ub, fb, dd);
+    R E[4];
+    if (shared) {
+      const R sg = a == 0 ? R(1) : R(-1);
+      for (int k = 0; k < 4; k++) E[k] = sg * E0[k];
+    } else {
+      face_vals(a, E);
+    }
+    const R EF = E[0], EB = E[1], ER = E[2], EL = E[3];
     R S = ((-B.sXc) * EF - (-B.sXb) * EB) * p.inv_dx + ((-B.sYc) * ER - (-B.sYl) * EL) * p.inv_dy;
     R div = R(2) * gam * S;
     R dw = R(0);
@@ -407,6 +503,7 @@ __device__ __forceinline__ void adj_phase_cell(const DevParams<R>& p, const R (&
       if (act[a]) tb -= g0[a] * dp0[a] / tau;
     if (pairtau) par[ADJ_TAU] += (double)tb;
     double gb = 0, db = 0;
+    R ebs[4] = {R(0), R(0), R(0), R(0)};  // combined seeds of the shared face terms
     for (int a = 0; a < 2; a++) {
       if (!act[a]) continue;
       const int b = 1 - a;
@@ -444,77 +541,33 @@ __device__ __forceinline__ void adj_phase_cell(const DevParams<R>& p, const R (&
         Bb.cYc += ub[1] * A.ph + wb[1];
         Ab.cYc -= ub[1] * B.ph;
       }
-      // div = 2 gam S, S = sum over faces of +-(-s) E / h
+      // div = 2 gam S, S = sum over faces of +-(-s) E / h; the seeds of the
+      // face terms do not depend on their values
       {
-        R uF[2] = {A.sXc * B.gXc - B.sXc * A.gXc, R(0.5) * (A.sXc * (B.cYc + B.cYf) - B.sXc * (A.cYc + A.cYf))};
-        R uB[2] = {A.sXb * B.gXb - B.sXb * A.gXb, R(0.5) * (A.sXb * (B.cYb + B.cYc) - B.sXb * (A.cYb + A.cYc))};
-        R uR[2] = {R(0.5) * (A.sYc * (B.cXc + B.cXr) - B.sYc * (A.cXc + A.cXr)), A.sYc * B.gYc - B.sYc * A.gYc};
-        R uL[2] = {R(0.5) * (A.sYl * (B.cXl + B.cXc) - B.sYl * (A.cXl + A.cXc)), A.sYl * B.gYl - B.sYl * A.gYl};
-        // the seeds of the face terms do not depend on their values, so one
-        // evaluation per face gives both the value and its adjoint
         const R Sb = divb * R(2) * gam;
         const R GF = Sb * p.inv_dx, GB = -Sb * p.inv_dx, GR = Sb * p.inv_dy, GL = -Sb * p.inv_dy;
-        R uFb[2] = {R(0), R(0)}, uBb[2] = {R(0), R(0)}, uRb[2] = {R(0), R(0)}, uLb[2] = {R(0), R(0)};
-        R fF = R(0), fB = R(0), fR = R(0), fL = R(0);
-        const R EF = face_E_rev(Mrot(a, b), Miv(a, b), dab, uF, uF[0], 0, -B.sXc * GF, uFb, fF, db);
-        const R EB = face_E_rev(Mrot(a, b), Miv(a, b), dab, uB, uB[0], 0, -B.sXb * GB, uBb, fB, db);
-        const R ER = face_E_rev(Mrot(a, b), Miv(a, b), dab, uR, uR[1], 1, -B.sYc * GR, uRb, fR, db);
-        const R EL = face_E_rev(Mrot(a, b), Miv(a, b), dab, uL, uL[1], 1, -B.sYl * GL, uLb, fL, db);
-        const R S = ((-B.sXc) * EF - (-B.sXb) * EB) * p.inv_dx + ((-B.sYc) * ER - (-B.sYl) * EL) * p.inv_dy;
+        const R eb[4] = {-B.sXc * GF, -B.sXb * GB, -B.sYc * GR, -B.sYl * GL};
+        R E[4];
+        if (shared) {
+          const R sg = a == 0 ? R(1) : R(-1);
+          for (int k = 0; k < 4; k++) {
+            E[k] = sg * E0[k];
+            ebs[k] += sg * eb[k];
+          }
+        } else {
+          faces_rev(Mrot(a, b), Miv(a, b), dab, A, B, Ab, Bb, eb, E, db);
+        }
+        const R S = ((-B.sXc) * E[0] - (-B.sXb) * E[1]) * p.inv_dx + ((-B.sYc) * E[2] - (-B.sYl) * E[3]) * p.inv_dy;
         gb += (double)(divb * R(2) * S);
-        Bb.sXc -= GF * EF;
-        Bb.sXb -= GB * EB;
-        Bb.sYc -= GR * ER;
-        Bb.sYl -= GL * EL;
-        uFb[0] += fF;
-        uBb[0] += fB;
-        uRb[1] += fR;
-        uLb[1] += fL;
-        // uF
-        Ab.sXc += uFb[0] * B.gXc;
-        Bb.gXc += uFb[0] * A.sXc;
-        Bb.sXc -= uFb[0] * A.gXc;
-        Ab.gXc -= uFb[0] * B.sXc;
-        Ab.sXc += uFb[1] * R(0.5) * (B.cYc + B.cYf);
-        Bb.cYc += uFb[1] * R(0.5) * A.sXc;
-        Bb.cYf += uFb[1] * R(0.5) * A.sXc;
-        Bb.sXc -= uFb[1] * R(0.5) * (A.cYc + A.cYf);
-        Ab.cYc -= uFb[1] * R(0.5) * B.sXc;
-        Ab.cYf -= uFb[1] * R(0.5) * B.sXc;
-        // uB
-        Ab.sXb += uBb[0] * B.gXb;
-        Bb.gXb += uBb[0] * A.sXb;
-        Bb.sXb -= uBb[0] * A.gXb;
-        Ab.gXb -= uBb[0] * B.sXb;
-        Ab.sXb += uBb[1] * R(0.5) * (B.cYb + B.cYc);
-        Bb.cYb += uBb[1] * R(0.5) * A.sXb;
-        Bb.cYc += uBb[1] * R(0.5) * A.sXb;
-        Bb.sXb -= uBb[1] * R(0.5) * (A.cYb + A.cYc);
-        Ab.cYb -= uBb[1] * R(0.5) * B.sXb;
-        Ab.cYc -= uBb[1] * R(0.5) * B.sXb;
-        // uR
-        Ab.sYc += uRb[0] * R(0.5) * (B.cXc + B.cXr);
-        Bb.cXc += uRb[0] * R(0.5) * A.sYc;
-        Bb.cXr += uRb[0] * R(0.5) * A.sYc;
-        Bb.sYc -= uRb[0] * R(0.5) * (A.cXc + A.cXr);
-        Ab.cXc -= uRb[0] * R(0.5) * B.sYc;
-        Ab.cXr -= uRb[0] * R(0.5) * B.sYc;
-        Ab.sYc += uRb[1] * B.gYc;
-        Bb.gYc += uRb[1] * A.sYc;
-        Bb.sYc -= uRb[1] * A.gYc;
-        Ab.gYc -= uRb[1] * B.sYc;
-        // uL
-        Ab.sYl += uLb[0] * R(0.5) * (B.cXl + B.cXc);
-        Bb.cXl += uLb[0] * R(0.5) * A.sYl;
-        Bb.cXc += uLb[0] * R(0.5) * A.sYl;
-        Bb.sYl -= uLb[0] * R(0.5) * (A.cXl + A.cXc);
-        Ab.cXl -= uLb[0] * R(0.5) * B.sYl;
-        Ab.cXc -= uLb[0] * R(0.5) * B.sYl;
-        Ab.sYl += uLb[1] * B.gYl;
-        Bb.gYl += uLb[1] * A.sYl;
-        Bb.sYl -= uLb[1] * A.gYl;
-        Ab.gYl -= uLb[1] * B.sYl;
+        Bb.sXc -= GF * E[0];
+        Bb.sXb -= GB * E[1];
+        Bb.sYc -= GR * E[2];
+        Bb.sYl -= GL * E[3];
       }
+    }
+    if (shared && (ebs[0] != R(0) || ebs[1] != R(0) || ebs[2] != R(0) || ebs[3] != R(0))) {
+      R E[4];
+      faces_rev(Mrot(0, 1), Miv(0, 1), dab, s[0], s[1], sb[0], sb[1], ebs, E, db);
     }
     par[ADJ_GAMMA] += gb;
     if (an) par[ADJ_DAB] += db;
