@@ -79,15 +79,92 @@ __device__ __forceinline__ void load_v(const DevParams<R>& p, const R* phi, int 
       for (int j = 0; j < 3; j++) v[a][i][j] = phi[idx(p, a, x + i - 1, y + j - 1)];
 }
 
-// reverse of the mu/phi update, one thread per interior cell; partials
-// gathered over the tile's footprint (phi_0, phi_1, mu, delta-phi_0, delta-phi_1)
+// fixed-order sum of the block partials into the gradient accumulator, by one
+// block of 256 threads
+__device__ __forceinline__ void adj_reduce_block(const double* __restrict__ part, int nblk, double* __restrict__ acc,
+                                                 double* red) {
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  double v[ADJ_NPAR] = {0, 0, 0, 0};
+  for (int b = t; b < nblk; b += 256)
+    for (int k = 0; k < ADJ_NPAR; k++) v[k] += part[(size_t)b * ADJ_NPAR + k];
+  for (int k = 0; k < ADJ_NPAR; k++) red[k * 256 + t] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+      for (int k = 0; k < ADJ_NPAR; k++) red[k * 256 + t] += red[k * 256 + t + s];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int k = 0; k < ADJ_NPAR; k++) acc[k] += red[k * 256];
+}
+
+// Sum of the partials pointing at cell (x, y) of [1, rows-2]^2: sp[0..1] for
+// phi, sm for mu (update footprints, then phase footprints).
+struct FpGrid {
+  int nbx_c, nby_c, nbx_f, nby_f;  // update tiles, phase tiles
+};
+
+template <class R>
+__device__ __forceinline__ void gather_at(const R* __restrict__ fpc, const R* __restrict__ fpp, const FpGrid& g, int x,
+                                          int y, R* sp, R& sm) {
+  R a[3], b[3];
+  fp_sum<R, FC_N>(fpc, x, y, 2, 2, g.nbx_c, g.nby_c, 0, 3, a);
+  fp_sum<R, FF_N>(fpp, x, y, 1, 1, g.nbx_f, g.nby_f, 0, 3, b);
+  sp[0] = a[0] + b[0];
+  sp[1] = a[1] + b[1];
+  sm = a[2] + b[2];
+}
+
+// New adjoint of interior cell (x, y): the partials pointing at it plus those
+// pointing at its ghost images (the reverse of the ghost copies, which write
+// an interior value to the cells write_with_ghosts names).
+template <class R>
+__device__ __forceinline__ void gather_lam(const DevParams<R>& p, const R* __restrict__ fpc, const R* __restrict__ fpp,
+                                           const FpGrid& g, int x, int y, R* sp, R& sm) {
+  gather_at(fpc, fpp, g, x, y, sp, sm);
+  // images: ghost rows/columns whose copy source is x / y, per field
+  for (int f = 0; f < 2; f++) {
+    int tx[3], ty[3], nx = 1, ny = 1;
+    tx[0] = x;
+    ty[0] = y;
+    for (int s = 0; s < 6; s++) {
+      if (p.gsrc[f][0][s] == x) tx[nx++] = ghost_index(s, p.rx);
+      if (p.gsrc[f][1][s] == y) ty[ny++] = ghost_index(s, p.ry);
+    }
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++) {
+        if (i == 0 && j == 0) continue;
+        const int gx = tx[i], gy = ty[j];
+        if (gx < 1 || gx > p.rx - 2 || gy < 1 || gy > p.ry - 2) continue;  // never read by a stencil
+        R gp[2], gm;
+        gather_at(fpc, fpp, g, gx, gy, gp, gm);
+        if (f == 0) {
+          sp[0] += gp[0];
+          sp[1] += gp[1];
+        } else {
+          sm += gm;
+        }
+      }
+  }
+}
+
+// Reverse of the mu/phi update, one thread per interior cell; partials
+// gathered over the tile's footprint (phi_0, phi_1, mu, delta-phi_0,
+// delta-phi_1).  The adjoint of a cell's new state is gathered from the
+// previous reverse step's footprints (fpc_prev, fpp_prev), or read from the
+// planes lam_phi / lam_mu for the first step; the first block also sums the
+// previous step's nprev parameter partials into acc.
 template <class R>
 __global__ void __launch_bounds__(256) GP_HEAVY k_adj_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                   const R* __restrict__ mu, const R* __restrict__ dphi,
                                                   const R* __restrict__ lam_phi, const R* __restrict__ lam_mu,
-                                                  R* __restrict__ fpc, double* __restrict__ part) {
+                                                  const R* __restrict__ fpc_prev, const R* __restrict__ fpp_prev,
+                                                  FpGrid g, const double* __restrict__ part_prev, int nprev,
+                                                  double* __restrict__ acc_par, R* __restrict__ fpc,
+                                                  double* __restrict__ part) {
   __shared__ double red[ADJ_NPAR * 256];
   __shared__ R acc[FC_N * FPN];
+  if (nprev > 0 && blockIdx.x == 0 && blockIdx.y == 0) adj_reduce_block(part_prev, nprev, acc_par, red);
   const DevParams<R>& p = *prm;
   const int tx = threadIdx.x, ty = threadIdx.y, t = ty * ABX + tx;
   const int y = 3 + blockIdx.x * ABX + tx;
@@ -96,9 +173,15 @@ __global__ void __launch_bounds__(256) GP_HEAVY k_adj_conc(const DevParams<R>* _
   double par[ADJ_NPAR] = {0, 0, 0, 0};
   R vb[2][3][3] = {}, mub5[5] = {}, dpb5[5][2] = {};
   if (x <= p.mesh_x + 2 && y <= p.mesh_y + 2) {
-    const size_t o = (size_t)x * p.pitch + y;
-    R seedp[2] = {lam_phi[o], lam_phi[p.plane + o]};
-    R seedm = lam_mu[o];
+    R seedp[2], seedm;
+    if (fpc_prev) {
+      gather_lam(p, fpc_prev, fpp_prev, g, x, y, seedp, seedm);
+    } else {
+      const size_t o = (size_t)x * p.pitch + y;
+      seedp[0] = lam_phi[o];
+      seedp[1] = lam_phi[p.plane + o];
+      seedm = lam_mu[o];
+    }
     if (seedp[0] != R(0) || seedp[1] != R(0) || seedm != R(0)) {
       R v[2][3][3], mu5[5], dp5[5][2];
       load_v(p, phi, x, y, v);
@@ -181,48 +264,8 @@ __global__ void __launch_bounds__(256) GP_HEAVY k_adj_phase(const DevParams<R>* 
     for (int k = 0; k < ADJ_NPAR; k++) part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * ADJ_NPAR + k] = par[k];
 }
 
-// fixed-order sum of the block partials into the gradient accumulator, by one
-// block of 256 threads
-__device__ __forceinline__ void adj_reduce_block(const double* __restrict__ part, int nblk, double* __restrict__ acc,
-                                                 double* red) {
-  const int t = threadIdx.y * blockDim.x + threadIdx.x;
-  double v[ADJ_NPAR] = {0, 0, 0, 0};
-  for (int b = t; b < nblk; b += 256)
-    for (int k = 0; k < ADJ_NPAR; k++) v[k] += part[(size_t)b * ADJ_NPAR + k];
-  for (int k = 0; k < ADJ_NPAR; k++) red[k * 256 + t] = v[k];
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s)
-      for (int k = 0; k < ADJ_NPAR; k++) red[k * 256 + t] += red[k * 256 + t + s];
-    __syncthreads();
-  }
-  if (t == 0)
-    for (int k = 0; k < ADJ_NPAR; k++) acc[k] += red[k * 256];
-}
-
-// Sum of the partials pointing at cell (x, y) of [1, rows-2]^2: sp[0..1] for
-// phi, sm for mu (update footprints, then phase footprints).
-struct FpGrid {
-  int nbx_c, nby_c, nbx_f, nby_f;  // update tiles, phase tiles
-};
-
-template <class R>
-__device__ __forceinline__ void gather_at(const R* __restrict__ fpc, const R* __restrict__ fpp, const FpGrid& g, int x,
-                                          int y, R* sp, R& sm) {
-  R a[3], b[3];
-  fp_sum<R, FC_N>(fpc, x, y, 2, 2, g.nbx_c, g.nby_c, 0, 3, a);
-  fp_sum<R, FF_N>(fpp, x, y, 1, 1, g.nbx_f, g.nby_f, 0, 3, b);
-  sp[0] = a[0] + b[0];
-  sp[1] = a[1] + b[1];
-  sm = a[2] + b[2];
-}
-
-// New adjoint of every interior cell: the partials pointing at it plus those
-// pointing at its ghost images (the reverse of the ghost copies, which write
-// an interior value to the cells write_with_ghosts names).
-// The first block also sums the parameter partials of this step's update and
-// phase reverses into acc (they are complete when the kernel starts; the work
-// overlaps the other blocks' gathers).
+// New adjoint of every interior cell into planes (after the last reverse
+// step); the first block also sums the last step's parameter partials.
 template <class R>
 __global__ void __launch_bounds__(256) k_adj_gather(const DevParams<R>* __restrict__ prm, const R* __restrict__ fpc,
                                                     const R* __restrict__ fpp, FpGrid g, R* __restrict__ lam_phi,
@@ -235,31 +278,7 @@ __global__ void __launch_bounds__(256) k_adj_gather(const DevParams<R>* __restri
   const int x = 3 + blockIdx.y * ABY + threadIdx.y;
   if (x > p.mesh_x + 2 || y > p.mesh_y + 2) return;
   R sp[2], sm;
-  gather_at(fpc, fpp, g, x, y, sp, sm);
-  // images: ghost rows/columns whose copy source is x / y, per field
-  for (int f = 0; f < 2; f++) {
-    int tx[3], ty[3], nx = 1, ny = 1;
-    tx[0] = x;
-    ty[0] = y;
-    for (int s = 0; s < 6; s++) {
-      if (p.gsrc[f][0][s] == x) tx[nx++] = ghost_index(s, p.rx);
-      if (p.gsrc[f][1][s] == y) ty[ny++] = ghost_index(s, p.ry);
-    }
-    for (int i = 0; i < nx; i++)
-      for (int j = 0; j < ny; j++) {
-        if (i == 0 && j == 0) continue;
-        const int gx = tx[i], gy = ty[j];
-        if (gx < 1 || gx > p.rx - 2 || gy < 1 || gy > p.ry - 2) continue;  // never read by a stencil
-        R gp[2], gm;
-        gather_at(fpc, fpp, g, gx, gy, gp, gm);
-        if (f == 0) {
-          sp[0] += gp[0];
-          sp[1] += gp[1];
-        } else {
-          sm += gm;
-        }
-      }
-  }
+  gather_lam(p, fpc, fpp, g, x, y, sp, sm);
   const size_t o = (size_t)x * p.pitch + y;
   lam_phi[o] = sp[0];
   lam_phi[p.plane + o] = sp[1];
@@ -276,10 +295,13 @@ struct gp_adjoint {
   bool direct;           // one slot per step: the forward steps slot k into slot k+1 and keeps delta-phi
   size_t dphi_off;       // byte offset of delta-phi in a slot (direct)
   std::vector<void *> slots;
-  void *fpc, *fpp;       // footprint sums of the update and phase reverse tiles
+  void *fpc[2], *fpp[2];  // footprint sums of the update and phase reverse tiles (this and the previous step)
+  double *partb[2];       // block parameter partials (this and the previous step)
+  int fcur;               // footprint / partial buffer of the next reverse step
+  bool have_prev;         // a reverse step has run since the seed
   void *lam[2][2];       // [buffer][0 = phi (2 planes), 1 = mu]
   int lcur;
-  double *part, *acc;
+  double *acc;
   int nblk;
   hipEvent_t e0, e1;
   double fwd_ms, rev_ms;
@@ -384,6 +406,7 @@ static int adj_seed(gp_adjoint *A) {
   A->loss = L;
   std::vector<R> zero((size_t)c->plane, R(0));
   A->lcur = 0;
+  A->have_prev = false;
   GP_HIP_CHECK(hipMemcpy(A->lam[0][0], seed.data(), seed.size() * sizeof(R), hipMemcpyHostToDevice));
   GP_HIP_CHECK(hipMemcpy(A->lam[0][1], zero.data(), zero.size() * sizeof(R), hipMemcpyHostToDevice));
   A->seeded = true;
@@ -414,18 +437,41 @@ static int adj_step(gp_adjoint *A, const void *phi_s, const void *mu_s, const vo
   dim3 ab(ABX, ABY);
   dim3 g_in((MY + ABX - 1) / ABX, (MX + ABY - 1) / ABY);
   dim3 g_ring((c->ry - 4 + ABX - 1) / ABX, (c->rx - 4 + ABY - 1) / ABY);
-  const int li = A->lcur, lo = A->lcur ^ 1;
   const int nb_in = (int)(g_in.x * g_in.y), nb_ring = (int)(g_ring.x * g_ring.y);
   const FpGrid fg = {(int)g_in.x, (int)g_in.y, (int)g_ring.x, (int)g_ring.y};
-  hipLaunchKernelGGL((k_adj_conc<R>), g_in, ab, 0, c->stream, prm, phi, mu, dphi,
-                     (const R *)A->lam[li][0], (const R *)A->lam[li][1], (R *)A->fpc, A->part);
-  hipLaunchKernelGGL((k_adj_phase<R>), g_ring, ab, 0, c->stream, prm, phi, mu, (const R *)A->fpc, fg.nbx_c, fg.nby_c,
-                     (R *)A->fpp, A->part + (size_t)nb_in * ADJ_NPAR);
-  hipLaunchKernelGGL((k_adj_gather<R>), g_in, ab, 0, c->stream, prm, (const R *)A->fpc, (const R *)A->fpp, fg,
-                     (R *)A->lam[lo][0], (R *)A->lam[lo][1], (const double *)A->part, nb_in + nb_ring, A->acc);
+  const int cb = A->fcur, pv = cb ^ 1;
+  const bool prev = A->have_prev;
+  hipLaunchKernelGGL((k_adj_conc<R>), g_in, ab, 0, c->stream, prm, phi, mu, dphi, (const R *)A->lam[A->lcur][0],
+                     (const R *)A->lam[A->lcur][1], prev ? (const R *)A->fpc[pv] : nullptr,
+                     prev ? (const R *)A->fpp[pv] : nullptr, fg, (const double *)A->partb[pv],
+                     prev ? nb_in + nb_ring : 0, A->acc, (R *)A->fpc[cb], A->partb[cb]);
+  hipLaunchKernelGGL((k_adj_phase<R>), g_ring, ab, 0, c->stream, prm, phi, mu, (const R *)A->fpc[cb], fg.nbx_c,
+                     fg.nby_c, (R *)A->fpp[cb], A->partb[cb] + (size_t)nb_in * ADJ_NPAR);
   GP_HIP_CHECK(hipGetLastError());
   if (timed) GP_HIP_CHECK(hipEventRecord(ep[1], c->stream));
-  A->lcur = lo;
+  A->fcur = pv;
+  A->have_prev = true;
+  return GP_OK;
+}
+
+// after the last reverse step: the adjoint of the initial state into the
+// planes lam[lcur] and the last step's parameter partials into acc
+template <class R>
+static int adj_finish(gp_adjoint *A) {
+  gp_ctx *c = A->c;
+  using namespace gpk;
+  if (!A->have_prev) return GP_OK;
+  const int MX = (int)c->hp.mesh_x, MY = (int)c->hp.mesh_y;
+  dim3 g_in((MY + ABX - 1) / ABX, (MX + ABY - 1) / ABY);
+  dim3 g_ring((c->ry - 4 + ABX - 1) / ABX, (c->rx - 4 + ABY - 1) / ABY);
+  const int nb_in = (int)(g_in.x * g_in.y), nb_ring = (int)(g_ring.x * g_ring.y);
+  const FpGrid fg = {(int)g_in.x, (int)g_in.y, (int)g_ring.x, (int)g_ring.y};
+  const int last = A->fcur ^ 1;
+  hipLaunchKernelGGL((k_adj_gather<R>), g_in, dim3(ABX, ABY), 0, c->stream, (const DevParams<R> *)c->dprm,
+                     (const R *)A->fpc[last], (const R *)A->fpp[last], fg, (R *)A->lam[A->lcur][0],
+                     (R *)A->lam[A->lcur][1], (const double *)A->partb[last], nb_in + nb_ring, A->acc);
+  GP_HIP_CHECK(hipGetLastError());
+  A->have_prev = false;
   return GP_OK;
 }
 
@@ -570,16 +616,18 @@ int gp_adjoint_create(gp_ctx *c, long nsteps, int nsnap, gp_adjoint **out) {
     const long MX = c->hp.mesh_x, MY = c->hp.mesh_y;
     const size_t nb_in = (size_t)((MY + gpk::ABX - 1) / gpk::ABX) * ((MX + gpk::ABY - 1) / gpk::ABY);
     const size_t nb_ring = (size_t)((c->ry - 4 + gpk::ABX - 1) / gpk::ABX) * ((c->rx - 4 + gpk::ABY - 1) / gpk::ABY);
-    ok = ok && hipMalloc(&A->fpc, nb_in * gpk::FC_N * gpk::FPN * c->esize) == hipSuccess &&
-         hipMalloc(&A->fpp, nb_ring * gpk::FF_N * gpk::FPN * c->esize) == hipSuccess;
+    for (int b = 0; b < 2 && ok; b++)
+      ok = hipMalloc(&A->fpc[b], nb_in * gpk::FC_N * gpk::FPN * c->esize) == hipSuccess &&
+           hipMalloc(&A->fpp[b], nb_ring * gpk::FF_N * gpk::FPN * c->esize) == hipSuccess;
   }
   for (int b = 0; b < 2 && ok; b++)
     ok = hipMalloc(&A->lam[b][0], pb * 2) == hipSuccess && hipMalloc(&A->lam[b][1], pb) == hipSuccess &&
          hipMemset(A->lam[b][0], 0, pb * 2) == hipSuccess && hipMemset(A->lam[b][1], 0, pb) == hipSuccess;
   const int gx = (c->ry - 2 + gpk::ABX - 1) / gpk::ABX, gy = (c->rx - 2 + gpk::ABY - 1) / gpk::ABY;
   A->nblk = 2 * gx * gy;  // partials of the update reverse and of the phase reverse
-  ok = ok && hipMalloc((void **)&A->part, sizeof(double) * gpk::ADJ_NPAR * A->nblk) == hipSuccess &&
-       hipMalloc((void **)&A->acc, sizeof(double) * gpk::ADJ_NPAR) == hipSuccess;
+  for (int b = 0; b < 2 && ok; b++)
+    ok = hipMalloc((void **)&A->partb[b], sizeof(double) * gpk::ADJ_NPAR * A->nblk) == hipSuccess;
+  ok = ok && hipMalloc((void **)&A->acc, sizeof(double) * gpk::ADJ_NPAR) == hipSuccess;
   ok = ok && hipEventCreate(&A->e0) == hipSuccess && hipEventCreate(&A->e1) == hipSuccess;
   // timing events, created here rather than during a run: two pairs in direct
   // mode, about one pair per reverse step and per recomputation otherwise
@@ -602,9 +650,10 @@ void gp_adjoint_destroy(gp_adjoint *A) {
   if (A->c) (void)hipStreamSynchronize(A->c->stream);
   for (void *s : A->slots)
     if (s) (void)hipFree(s);
-  for (void *p : {A->fpc, A->fpp, A->lam[0][0], A->lam[0][1], A->lam[1][0], A->lam[1][1]})
+  for (void *p : {A->fpc[0], A->fpc[1], A->fpp[0], A->fpp[1], A->lam[0][0], A->lam[0][1], A->lam[1][0], A->lam[1][1]})
     if (p) (void)hipFree(p);
-  if (A->part) (void)hipFree(A->part);
+  for (double *p : A->partb)
+    if (p) (void)hipFree(p);
   if (A->acc) (void)hipFree(A->acc);
   if (A->e0) (void)hipEventDestroy(A->e0);
   if (A->e1) (void)hipEventDestroy(A->e1);
@@ -636,6 +685,7 @@ int gp_adjoint_run(gp_adjoint *A, const double *phi0, const double *mu0, const d
     rc = adj_reverse(A, 0, A->nsteps, 0, pool);
   }
   if (rc) return rc;
+  if ((rc = c->dtype == GP_F64 ? adj_finish<double>(A) : adj_finish<float>(A))) return rc;
   GP_HIP_CHECK(hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < A->ev_kind.size(); i++) {
     float ms = 0;
