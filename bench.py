@@ -158,6 +158,26 @@ def adjoint_report(a):
     print(json.dumps(out), flush=True)
 
 
+def adjoint_time(a):
+    """One adjoint run (all states kept) through the dendrite workload in --dtype
+    for --steps steps (default 500), loss against the initial solid fraction;
+    prints the wall and device times (a short run for kernel profiles)."""
+    from eshwar_dipan_amd import gpstep, workloads
+    inp, phi0, mu0, nsteps_full, desc = workloads.WORKLOADS[a.workload or "c5"]()
+    steps = a.steps or 500
+    s = gpstep.Solver(inp.p, a.dtype)
+    adj = gpstep.Adjoint(s, steps, 0)
+    t0 = time.perf_counter()
+    loss, grad, _ = adj.run(phi0, mu0, phi0[0].copy())
+    t_adj = time.perf_counter() - t0
+    f_ms, r_ms, nfwd = adj.info()
+    adj.close()
+    s.close()
+    print(json.dumps({"metric": "adjoint run time", "config": {"workload": desc}, "dtype": a.dtype, "steps": steps,
+                      "adjoint_s": t_adj, "device_fwd_ms": f_ms, "device_rev_ms": r_ms, "loss": loss, "grad": grad}),
+          flush=True)
+
+
 def bench_ensemble(a):
     """Ensemble workload: every member grid stepped together as one tile set."""
     import torch
@@ -237,16 +257,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy five times (calibration of the HBM byte counters)")
-    ap.add_argument("--report", choices=["precision", "adjoint"], default=None,
+    ap.add_argument("--report", choices=["precision", "adjoint", "adjoint-time"], default=None,
                     help="precision: binary32 against binary64 on --workload (default c3), per-field error, "
                          "solute drift and time-to-solution ratio over --steps (default 2000); adjoint: gradients "
-                         "through the dendrite workload (default c5) against central differences, adjoint/forward time")
+                         "through the dendrite workload (default c5) against central differences, adjoint/forward time; "
+                         "adjoint-time: one adjoint run of --steps (default 500) steps, for kernel profiles")
     a = ap.parse_args()
     if a.report == "precision":
         precision_report(a)
         return
     if a.report == "adjoint":
         adjoint_report(a)
+        return
+    if a.report == "adjoint-time":
+        adjoint_time(a)
         return
     if a.copy_probe:
         import torch
