@@ -154,8 +154,14 @@ __device__ __forceinline__ void gather_lam(const DevParams<R>& p, const R* __res
 // previous reverse step's footprints (fpc_prev, fpp_prev), or read from the
 // planes lam_phi / lam_mu for the first step; the first block also sums the
 // previous step's nprev parameter partials into acc.
+// Waves per SIMD of the update reverse: in binary64 it needs about 170
+// registers, and a bound of 3 waves (168) beats 2 by 6% at 512^2; the
+// binary32 kernel fits 5 waves either way.
+#ifndef GP_ADJ_CONC_WAVES
+#define GP_ADJ_CONC_WAVES 3
+#endif
 template <class R>
-__global__ void __launch_bounds__(256) GP_HEAVY k_adj_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GP_ADJ_CONC_WAVES))) k_adj_conc(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                   const R* __restrict__ mu, const R* __restrict__ dphi,
                                                   const R* __restrict__ lam_phi, const R* __restrict__ lam_mu,
                                                   const R* __restrict__ fpc_prev, const R* __restrict__ fpp_prev,

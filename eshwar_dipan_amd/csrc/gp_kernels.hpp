@@ -24,6 +24,9 @@
 #define GP_WAVES_PER_EU 2
 #endif
 #define GP_HEAVY __attribute__((amdgpu_waves_per_eu(GP_WAVES_PER_EU)))
+#ifndef GP_CONC_WAVES
+#define GP_CONC_WAVES GP_WAVES_PER_EU
+#endif
 
 namespace gpk {
 
@@ -242,7 +245,7 @@ constexpr int KM = GP_KM;  // rows per wave
 constexpr int MW = 63;    // cells per wave along y
 
 template <int P, int C, class R>
-__global__ void __launch_bounds__(BX * BY) GP_HEAVY k_conc_march(const DevParams<R>* __restrict__ prm,
+__global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP_CONC_WAVES))) k_conc_march(const DevParams<R>* __restrict__ prm,
                                                               const R* __restrict__ phi, const R* __restrict__ mu,
                                                               const R* __restrict__ dphi, const R* __restrict__ mulo,
                                                               R* __restrict__ phi_o, R* __restrict__ mu_o,
@@ -332,7 +335,7 @@ __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_conc_march(const DevParams
 // w > 0 is read from row w-1 after a barrier.  1 + 1/BY + 64/63 face
 // evaluations per cell.
 template <int P, int C, class R>
-__global__ void __launch_bounds__(BX * BY) GP_HEAVY k_conc_lds(const DevParams<R>* __restrict__ prm,
+__global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP_CONC_WAVES))) k_conc_lds(const DevParams<R>* __restrict__ prm,
                                                             const R* __restrict__ phi, const R* __restrict__ mu,
                                                             const R* __restrict__ dphi, const R* __restrict__ mulo,
                                                             R* __restrict__ phi_o, R* __restrict__ mu_o,
