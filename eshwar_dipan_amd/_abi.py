@@ -51,6 +51,11 @@ class GpParams(ct.Structure):
         ("dBbdT", _arr(_D, PM, CM)),
         ("B", _arr(_D, PM, CM)),
         ("Cf", _arr(_D, PM)),
+        ("delta_T_s", _arr(_D, PM)), ("delta_T_l", _arr(_D, PM)),
+        ("delta_C", _arr(_D, PM, CM)),
+        ("tempgrad", _arr(_D, 5)),
+        ("bc_type_T", _arr(ct.c_int, 4)),
+        ("bc_proxy_T", _arr(ct.c_long, 4, 3)),
     ]
 
 
@@ -88,6 +93,7 @@ class GpInput(ct.Structure):
         ("have_dab", ct.c_int), ("have_gamma_abc", ct.c_int),
         ("rot_given", _arr(ct.c_int, PM, PM)),
         ("bc_proxy_set", _arr(ct.c_int, 2, 4)),
+        ("bc_proxy_set_T", _arr(ct.c_int, 4)),
         ("ncomponents_names", ct.c_int), ("nphases_names", ct.c_int),
         ("phases", _arr(ct.c_char, PM, GP_NAME_LEN)),
         ("components", _arr(ct.c_char, CM + 1, GP_NAME_LEN)),

@@ -604,7 +604,7 @@ extern "C" {
 int gp_adjoint_create(gp_ctx *c, long nsteps, int nsnap, gp_adjoint **out) {
   if (!c || !out || nsteps < 1 || nsnap < 0 || c->nmember) return GP_E_ARG;
   *out = nullptr;
-  if (c->P != 2 || c->C != 1 || c->hp.mu_solver != GP_MU_BINARY) return GP_E_UNSUPPORTED;
+  if (c->P != 2 || c->C != 1 || c->hp.mu_solver != GP_MU_BINARY || !c->hp.isothermal) return GP_E_UNSUPPORTED;
   gp_adjoint *A = new (std::nothrow) gp_adjoint();
   if (!A) return GP_E_NOMEM;
   A->c = c;

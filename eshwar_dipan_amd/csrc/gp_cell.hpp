@@ -51,6 +51,14 @@ struct DevParams {
   R Bsh[PM][CM];          // Beq + dBbdT*(T-Teq)
   R B[PM][CM];
   R Cf[PM];
+  // non-isothermal runs: the temperature of padded row Y is tb + Y*tgG, tb per step
+  int ni;
+  int tsrc[6];            // y ghost rows 0,1,2, ry-3, ry-2, ry-1 -> row their temperature copy reads (-1: none)
+  R tgG, tgv;             // DeltaT*dy/Distance, velocity (Tempgrady)
+  R Teq;
+  R Beq[PM][CM], dBbdT[PM][CM];
+  R dTs[PM], dTl[PM], dC[PM][CM];   // DELTA_T[a][a], DELTA_T[a][liquid], DELTA_C[a]
+  R ceqL[PM][CM], ceqS[PM][CM];     // ceq[a][liquid], ceq[a][a]
 };
 
 template <class R>
@@ -184,6 +192,117 @@ __device__ __forceinline__ R tau_phi(const DevParams<R>& p, const R* ph) {
       }
   if (s1 != R(0)) return s / s1;
   return p.tau;
+}
+
+// ---------------------------------------------------------------- temperature
+
+// Non-isothermal runs (ISOTHERMAL = 0): temperature of padded row y,
+// temp_bottom + Y*GRADIENT (functions/Temperature_gradient.h:17-19), where Y is
+// the row itself or, when proxy is set and the row is a ghost row with a
+// temperature boundary copy, the row it copies (boundary_serial.h:176-203).
+template <class R>
+__device__ __forceinline__ R temp_row(const DevParams<R>& p, R tb, int y, bool proxy) {
+  int Y = y;
+  if (proxy) {
+    if (y <= 2 && p.tsrc[y] >= 0) Y = p.tsrc[y];
+    else if (y >= p.ry - 3 && p.tsrc[3 + y - (p.ry - 3)] >= 0) Y = p.tsrc[3 + y - (p.ry - 3)];
+  }
+  return tb + (R)Y * p.tgG;
+}
+
+// Beq + dBbdT (T - Teq) of c_a(mu) and Mu_a(c) at temperature T (functionF_01.h:19-37)
+template <class R>
+__device__ __forceinline__ R bsh_t(const DevParams<R>& p, int a, int k, R T) {
+  return p.Beq[a][k] + p.dBbdT[a][k] * (T - p.Teq);
+}
+
+template <int C, class R>
+__device__ __forceinline__ R c_mu_t(const DevParams<R>& p, const R* mu, int a, int i, R T) {
+  R s = R(0);
+#pragma unroll
+  for (int k = 0; k < C; k++) s += p.cmu[a][i][k] * (mu[k] - bsh_t(p, a, k, T));
+  return s;
+}
+
+template <int C, class R>
+__device__ __forceinline__ R Mu_of_c_t(const DevParams<R>& p, const R* c, int a, int i, R T) {
+  R s = R(2.0) * p.A[a][i][i] * c[i] + bsh_t(p, a, i, T);
+#pragma unroll
+  for (int j = 0; j < C; j++)
+    if (i != j) s += p.A[a][i][j] * c[j];
+  return s;
+}
+
+// B_a,i(T) and C_a(T) (functionF_01.h:62-104), the tables init_propertymatrices(T)
+// leaves for the phase-field terms of a cell of temperature T
+template <int C, class R>
+__device__ __forceinline__ R B_of_T(const DevParams<R>& p, int L, R T, int i, int a) {
+  if (a == L) return R(0);
+  R cl[C], cs[C], sum_c = R(0);
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    cl[k] = p.ceqL[a][k] - (p.dC[a][k]) * (p.Teq - T) / (p.dTl[a]);
+    cs[k] = p.ceqS[a][k] - (p.dC[a][k]) * (p.Teq - T) / (p.dTs[a]);
+    if (k != i) sum_c += p.A[L][k][i] * cl[k] - p.A[a][k][i] * cs[k];
+  }
+  return (R(2.0) * (p.A[L][i][i] * cl[i] - p.A[a][i][i] * cs[i]) + sum_c);
+}
+
+template <int C, class R>
+__device__ __forceinline__ R C_of_T(const DevParams<R>& p, int L, R T, int a) {
+  if (a == L) return R(0);
+  R cl[C], cs[C], sum_c = R(0);
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    cl[k] = p.ceqL[a][k] - (p.dC[a][k]) * (p.Teq - T) / (p.dTl[a]);
+    cs[k] = p.ceqS[a][k] - (p.dC[a][k]) * (p.Teq - T) / (p.dTs[a]);
+  }
+#pragma unroll
+  for (int i = 0; i < C; i++)
+#pragma unroll
+    for (int j = 0; j < C; j++)
+      if (i <= j) sum_c += (p.A[a][i][j] * cs[i] * cs[j] - p.A[L][i][j] * cl[i] * cl[j]);
+  return sum_c;
+}
+
+template <int P, int C, class R>
+__device__ __forceinline__ R psi_of_mu_t(const DevParams<R>& p, const R* mu, int b, R T) {
+  R c[C];
+  R psi = R(0);
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    c[k] = c_mu_t<C>(p, mu, b, k, T);
+    psi -= mu[k] * c[k];
+  }
+  // free energy with B(T), C(T)
+  R f = R(0);
+#pragma unroll
+  for (int i = 0; i < C; i++) {
+#pragma unroll
+    for (int j = 0; j < C; j++)
+      if (i <= j) f += p.A[b][i][j] * c[i] * c[j];
+    f += B_of_T<C>(p, P - 1, T, i, b) * c[i];
+  }
+  f += C_of_T<C>(p, P - 1, T, b);
+  psi += f;
+  return psi;
+}
+
+// the same functions at the run's temperature (tables) or, with NI, at T
+template <int C, bool NI, class R>
+__device__ __forceinline__ R c_mu_(const DevParams<R>& p, const R* mu, int a, int i, R T) {
+  if constexpr (NI) return c_mu_t<C>(p, mu, a, i, T);
+  else return c_mu<C>(p, mu, a, i);
+}
+template <int C, bool NI, class R>
+__device__ __forceinline__ R Mu_of_c_(const DevParams<R>& p, const R* c, int a, int i, R T) {
+  if constexpr (NI) return Mu_of_c_t<C>(p, c, a, i, T);
+  else return Mu_of_c<C>(p, c, a, i);
+}
+template <int P, int C, bool NI, class R>
+__device__ __forceinline__ R psi_of_mu_(const DevParams<R>& p, const R* mu, int b, R T) {
+  if constexpr (NI) return psi_of_mu_t<P, C>(p, mu, b, T);
+  else return psi_of_mu<C>(p, mu, b);
 }
 
 // ---------------------------------------------------------------- anisotropy
@@ -574,15 +693,16 @@ __device__ __forceinline__ bool cell_active(const DevParams<R>& p, const Nbhd<P,
 // delta-phi of one cell from its active set; with `pairs`, dA / dG hold the
 // anisotropic dA/dphi and div(dA/dgrad phi) of every active phase, else they
 // are evaluated here phase by phase  (calculate_divergence_phasefield.h:4-62)
-template <int P, int C, class R>
+template <int P, int C, class R, bool NI = false>
 __device__ __forceinline__ void cell_dphi_rest(const DevParams<R>& p, const Nbhd<P, R>& n, const R* mu,
-                                               const bool* act, bool any, bool pairs, R* dA, R* dG, R* dp) {
+                                               const bool* act, bool any, bool pairs, R* dA, R* dG, R* dp,
+                                               R T = R(0)) {
   R lam[P], psi[P];
 #pragma unroll
   for (int a = 0; a < P; a++) dp[a] = R(0);
   if (any) {
 #pragma unroll
-    for (int b = 0; b < P; b++) psi[b] = psi_of_mu<C>(p, mu, b);
+    for (int b = 0; b < P; b++) psi[b] = psi_of_mu_<P, C, NI>(p, mu, b, T);
     R sl = R(0);
     int active = 0;
 #pragma unroll
@@ -610,14 +730,16 @@ __device__ __forceinline__ void cell_dphi_rest(const DevParams<R>& p, const Nbhd
 }
 
 // delta-phi of one cell  (calculate_divergence_phasefield.h:4-62)
-template <int P, int C, class R>
-__device__ __forceinline__ void cell_dphi(const DevParams<R>& p, const Nbhd<P, R>& n, const R* mu, R* dp) {
+// (T: the cell's temperature, used with NI only)
+template <int P, int C, class R, bool NI = false>
+__device__ __forceinline__ void cell_dphi(const DevParams<R>& p, const Nbhd<P, R>& n, const R* mu, R* dp,
+                                          R T = R(0)) {
   bool act[P];
   const bool any = cell_active<P>(p, n, act);
   const bool pairs = p.anisotropy && p.pair_sym;
   R dA[P], dG[P];
   if (any && pairs) aniso_pairs<P>(p, n, act, dA, dG);
-  cell_dphi_rest<P, C>(p, n, mu, act, any, pairs, dA, dG, dp);
+  cell_dphi_rest<P, C, R, NI>(p, n, mu, act, any, pairs, dA, dG, dp, T);
 }
 
 // ---------------------------------------------------------------- mu update
@@ -650,10 +772,12 @@ __device__ __forceinline__ void cell_D(const DevParams<R>& p, const R* ph, R (&D
 // (calculate_fluxes_concentration.h:25-142).  g_p: normal gradients at the face,
 // ct_p / ct_q: tangential centred gradients at p and q (the face's tangential
 // gradient is their mean), phase fields, potentials and delta-phi of both cells.
-template <int P, int C, class R>
+// With NI, c(mu) of the two cells is taken at their temperatures T_p, T_q.
+template <int P, int C, class R, bool NI = false>
 __device__ __forceinline__ void face_jat(const DevParams<R>& p, const R* g_p, const R* ct_p, const R* ct_q,
                                          const R* ph_p, const R* ph_q, const R* mu_p, const R* mu_q,
-                                         const R* dp_p, const R* dp_q, bool enabled, R* jat) {
+                                         const R* dp_p, const R* dp_q, bool enabled, R* jat, R T_p = R(0),
+                                         R T_q = R(0)) {
   constexpr int L = P - 1;
 #pragma unroll
   for (int k = 0; k < C; k++) jat[k] = R(0);
@@ -675,8 +799,8 @@ __device__ __forceinline__ void face_jat(const DevParams<R>& p, const R* g_p, co
       if (ngl > R(0)) sp /= (ngl * ngp);
 #pragma unroll
       for (int k = 0; k < C; k++) {
-        R clq = c_mu<C>(p, mu_q, L, k), clp = c_mu<C>(p, mu_p, L, k);
-        R csq = c_mu<C>(p, mu_q, a, k), csp = c_mu<C>(p, mu_p, a, k);
+        R clq = c_mu_<C, NI>(p, mu_q, L, k, T_q), clp = c_mu_<C, NI>(p, mu_p, L, k, T_p);
+        R csq = c_mu_<C, NI>(p, mu_q, a, k, T_q), csp = c_mu_<C, NI>(p, mu_p, a, k, T_p);
         if (!p.obstacle)
           jat[k] += p.at_coef[a][k] * fabs(sp) * p.well_at * ((clq - csq) * dp_q[a] + (clp - csp) * dp_p[a]) * g_p[a] / ngp;
         else
@@ -767,13 +891,13 @@ __device__ __forceinline__ void lu_inverse(R (&a)[C][C], R (&inv)[C][C]) {
 // (calculate_divergence_concentration.h:4-153).  jat*/Dm*/gm*: antitrapping flux,
 // face mobility and mu gradient on the faces front (x|x+1), back (x-1|x),
 // right (y|y+1), left (y-1|y).
-template <int P, int C, class R>
+template <int P, int C, class R, bool NI = false>
 __device__ __forceinline__ void cell_update(const DevParams<R>& p, const R* ph, const R* mu, const R* dp,
                                             const R* jf, const R* jb, const R* jr, const R* jl,
                                             const R (&Dmf)[C][C], const R (&Dmb)[C][C],
                                             const R (&Dmr)[C][C], const R (&Dml)[C][C],
                                             const R* gmf, const R* gmb, const R* gmr, const R* gml,
-                                            R* ph_o, R* mu_o, R* dmu) {
+                                            R* ph_o, R* mu_o, R* dmu, R T = R(0)) {
   R divflux[C], divjat[C];
 #pragma unroll
   for (int k = 0; k < C; k++) {
@@ -798,10 +922,10 @@ __device__ __forceinline__ void cell_update(const DevParams<R>& p, const R* ph, 
     for (int a = 0; a < P; a++)
       if (a == bulk) {
 #pragma unroll
-        for (int k = 0; k < C; k++) cn[k] = c_mu<C>(p, mu, a, k) + p.dt * (divflux[k]) - divjat[k];
+        for (int k = 0; k < C; k++) cn[k] = c_mu_<C, NI>(p, mu, a, k, T) + p.dt * (divflux[k]) - divjat[k];
 #pragma unroll
         for (int k = 0; k < C; k++) {
-          mu_o[k] = Mu_of_c<C>(p, cn, a, k);
+          mu_o[k] = Mu_of_c_<C, NI>(p, cn, a, k, T);
           // the same update as an increment of the linear map Mu(c): dMu/dc . delta_c
           R d = R(0);
 #pragma unroll
@@ -819,12 +943,23 @@ __device__ __forceinline__ void cell_update(const DevParams<R>& p, const R* ph, 
 #pragma unroll
       for (int b = 0; b < P; b++) sdh += dhphi<P>(ph, a, b) * dp[b];
 #pragma unroll
-      for (int k = 0; k < C; k++) sum[k] += c_mu<C>(p, mu, a, k) * sdh;
+      for (int k = 0; k < C; k++) sum[k] += c_mu_<C, NI>(p, mu, a, k, T) * sdh;
       h[a] = hphi<P>(ph, a);
     }
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      dc[k] = p.dt * (divflux[k]) - divjat[k] - sum[k];
+      if constexpr (NI) {
+        // dcbdT_phase is allocated zero and never set, nor is the GRADIENT member
+        // of temperature_gradientY, so the term is a signed zero
+        // (calculate_divergence_concentration.h:22-40,85)
+        R sdc = R(0);
+#pragma unroll
+        for (int a = 0; a < P; a++) sdc += R(0) * h[a];
+        const R gm = R(0);
+        dc[k] = p.dt * (divflux[k]) - divjat[k] - sum[k] - sdc * p.dt * (-gm * p.tgv);
+      } else {
+        dc[k] = p.dt * (divflux[k]) - divjat[k] - sum[k];
+      }
 #pragma unroll
       for (int l = 0; l < C; l++) {
         dcdmu[k][l] = R(0);

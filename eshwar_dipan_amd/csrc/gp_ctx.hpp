@@ -25,6 +25,7 @@ struct gp_ctx {
   int nstat_blocks;
   int cur;              // index of the current state buffers
   int stepped;          // the other buffers hold the state one step earlier
+  long tstep;           // steps since the state was loaded (temperature of non-isothermal runs)
   int path;             // 0 fused, 1 two-pass
   hipStream_t stream;
   hipEvent_t ev0, ev1;

@@ -220,6 +220,11 @@ int gp_derive(gp_params *p) {
     }
   }
   for (int k = 0; k < C; k++) p->dBbdT[L][k] = 0.0;
+  for (int a = 0; a < L; a++) {
+    p->delta_T_s[a] = t.DT[a][a];
+    p->delta_T_l[a] = t.DT[a][L];
+    for (int k = 0; k < C; k++) p->delta_C[a][k] = t.DC[a][k];
+  }
   for (int a = 0; a < P; a++)
     for (int i = 0; i < C; i++) p->Beq[a][i] = fn_B(p, &t, p->Teq, i, a);
   for (int a = 0; a < P; a++) {
@@ -413,7 +418,7 @@ int gp_read_input(const char *path, gp_input *in, int verbose, char *err, size_t
     }
     else if (!strcmp(key, "Tempgrady") && in->tempgrady) {
       toks t; tokenize(val, &t);
-      for (int i = 0; i < 5 && i < t.n; i++) in->tempgrad[i] = atof(t.s[i]);
+      for (int i = 0; i < 5 && i < t.n; i++) in->tempgrad[i] = p->tempgrad[i] = atof(t.s[i]);
     }
     else if (!strcmp(key, "Shift")) in->shift = atoi(val);
     else if (!strcmp(key, "Shiftj")) { if (in->shift) in->shiftj = atol(val); }
@@ -491,7 +496,7 @@ int gp_read_input(const char *path, gp_input *in, int verbose, char *err, size_t
       if (ty[2] == 3 || ty[3] == 3) ty[2] = ty[3] = 3;
       for (int fc = 0; fc < 4; fc++) {
         if (fld < 2) p->bc_type[fld][fc] = ty[fc];
-        else in->bc_type_T[fc] = ty[fc];
+        else in->bc_type_T[fc] = p->bc_type_T[fc] = ty[fc];
       }
     } else if (!strcmp(key, "BOUNDARY_VALUE")) {
       toks t; tokenize(val, &t);
@@ -502,16 +507,20 @@ int gp_read_input(const char *path, gp_input *in, int verbose, char *err, size_t
       in->bc_value[fld][GP_FACE_XMAX] = atof(t.s[1]);
       in->bc_value[fld][GP_FACE_YMAX] = atof(t.s[3]);
       in->bc_value[fld][GP_FACE_YMIN] = atof(t.s[4]);
-      if (fld == 2) continue;
       /* proxies follow the face type at this point of the file */
       for (int fc = 0; fc < 4; fc++) {
-        int ty = p->bc_type[fld][fc];
+        int ty = fld < 2 ? p->bc_type[fld][fc] : p->bc_type_T[fc];
         long n = (fc == GP_FACE_XMIN || fc == GP_FACE_XMAX) ? rows[0] : rows[1];
         int low = (fc == GP_FACE_XMIN || fc == GP_FACE_YMIN);
         if (ty == GP_BC_NEUMANN || ty == GP_BC_PERIODIC) {
           int mirror_low = (ty == GP_BC_NEUMANN) == low;   /* proxies 4,5,6 */
-          for (int j = 0; j < 3; j++) p->bc_proxy[fld][fc][j] = mirror_low ? 4 + j : n - 5 - j;
-          in->bc_proxy_set[fld][fc] = 1;
+          for (int j = 0; j < 3; j++) {
+            long v = mirror_low ? 4 + j : n - 5 - j;
+            if (fld < 2) p->bc_proxy[fld][fc][j] = v;
+            else p->bc_proxy_T[fc][j] = v;
+          }
+          if (fld < 2) in->bc_proxy_set[fld][fc] = 1;
+          else in->bc_proxy_set_T[fc] = 1;
         }
       }
     }
@@ -530,7 +539,17 @@ int gp_check_supported(const gp_params *p, char *err, size_t errlen) {
   if (p->nphases < 2 || p->nphases > P_MAX || p->ncomp < 1 || p->ncomp > C_MAX) { set_err(err, errlen, "%s", "NUMPHASES/NUMCOMPONENTS out of range"); return GP_E_UNSUPPORTED; }
   if (p->mesh_x < 4 || p->mesh_y < 4) { set_err(err, errlen, "%s", "MESH_X and MESH_Y must be >= 4"); return GP_E_ARG; }
   if (!(p->dx > 0) || !(p->dy > 0) || !(p->dt > 0) || !(p->epsilon != 0)) { set_err(err, errlen, "%s", "DELTA_X, DELTA_Y, DELTA_t must be > 0 and epsilon != 0"); return GP_E_ARG; }
-  if (!p->isothermal) { set_err(err, errlen, "%s", "non-isothermal (Tempgrady) runs are not stepped by this build"); return GP_E_UNSUPPORTED; }
+  if (!p->isothermal) {
+    if (!(p->tempgrad[2] != 0) || !isfinite(p->tempgrad[0])) { set_err(err, errlen, "%s", "non-isothermal run needs Tempgrady = {base, DeltaT, Distance != 0, offset, velocity}"); return GP_E_ARG; }
+    for (int fc = GP_FACE_YMAX; fc <= GP_FACE_YMIN; fc++) {
+      int ty = p->bc_type_T[fc];
+      if (ty != GP_BC_NEUMANN && ty != GP_BC_PERIODIC) continue;   /* other types copy nothing */
+      for (int j = 0; j < 3; j++) {
+        long q = p->bc_proxy_T[fc][j];
+        if (q < 3 || q > p->mesh_y + 2) { set_err(err, errlen, "%s", "temperature boundary proxy outside the interior (BOUNDARY_VALUE {T, ...} missing?)"); return GP_E_ARG; }
+      }
+    }
+  }
   if (p->anisotropy != 0 && p->anisotropy != 1) { set_err(err, errlen, "%s", "Function_anisotropy must be 0 or 1"); return GP_E_UNSUPPORTED; }
   if (p->mu_solver < 0 || p->mu_solver > 3) { set_err(err, errlen, "%s", "bad mu_solver"); return GP_E_ARG; }
   if (p->mu_solver == GP_MU_BINARY && p->ncomp != 1) { /* the reference updates only mu[0]; keep that */ }

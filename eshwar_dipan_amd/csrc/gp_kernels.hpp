@@ -84,9 +84,9 @@ __device__ __forceinline__ void load3x3(const DevParams<R>& p, const R* __restri
 }
 
 // delta-phi of padded cell (x,y) into dphi (calculate_divergence_phasefield_2D)
-template <int P, int C, class R>
+template <int P, int C, class R, bool NI = false>
 __device__ __forceinline__ void phase_cell(const DevParams<R>& p, const R* __restrict__ phi, const R* __restrict__ mu,
-                                           R* __restrict__ dphi, int x, int y) {
+                                           R* __restrict__ dphi, int x, int y, R T = R(0)) {
   R v[P][3][3];
   load3x3<P>(p, phi, x, y, v);
   Nbhd<P, R> n;
@@ -94,7 +94,7 @@ __device__ __forceinline__ void phase_cell(const DevParams<R>& p, const R* __res
   R m[C], dp[P];
 #pragma unroll
   for (int k = 0; k < C; k++) m[k] = mu[idx(p, k, x, y)];
-  cell_dphi<P, C>(p, n, m, dp);
+  cell_dphi<P, C, R, NI>(p, n, m, dp, T);
 #pragma unroll
   for (int a = 0; a < P; a++) dphi[idx(p, a, x, y)] = dp[a];
 }
@@ -104,12 +104,12 @@ __device__ __forceinline__ void phase_cell(const DevParams<R>& p, const R* __res
 // faces; writes the cell and its ghost images.  Kahan adds the mu increment to
 // mu + mulo (binary32).  Returns the one-hot phase of the updated phi (-1 if
 // not one-hot).
-template <int P, int C, class R, bool Kahan>
+template <int P, int C, class R, bool Kahan, bool NI = false>
 __device__ __forceinline__ int conc_apply(const DevParams<R>& p, const R (&v)[P][3][3], const R* mc, const R* mf,
                                           const R* mb, const R* mr, const R* ml, const R* dc, const R* jf,
                                           const R* jb, const R* jr, const R* jl, const R* __restrict__ mulo,
                                           R* __restrict__ phi_o, R* __restrict__ mu_o, R* __restrict__ mulo_o, int x,
-                                          int y) {
+                                          int y, R T = R(0)) {
   R phc[P], phf[P], phb[P], phr[P], phl[P];
 #pragma unroll
   for (int a = 0; a < P; a++) {
@@ -142,7 +142,7 @@ __device__ __forceinline__ int conc_apply(const DevParams<R>& p, const R (&v)[P]
     gml[k] = div_dy(mc[k] - ml[k], p);
   }
   R pho[P], muo[C], dmu[C];
-  cell_update<P, C>(p, phc, mc, dc, jf, jb, jr, jl, Dmf, Dmb, Dmr, Dml, gmf, gmb, gmr, gml, pho, muo, dmu);
+  cell_update<P, C, R, NI>(p, phc, mc, dc, jf, jb, jr, jl, Dmf, Dmb, Dmr, Dml, gmf, gmb, gmr, gml, pho, muo, dmu, T);
 #pragma unroll
   for (int a = 0; a < P; a++) write_with_ghosts(p, 0, phi_o + (size_t)a * p.plane, x, y, pho[a]);
   if constexpr (!Kahan) {
@@ -214,16 +214,20 @@ __device__ __forceinline__ int conc_cell(const DevParams<R>& p, const R* __restr
 // delta-phi on [2, rows-3]^2, one cell per thread
 __host__ __device__ inline dim3 phase_grid(int rx, int ry) { return dim3((ry - 4 + BX - 1) / BX, (rx - 4 + BY - 1) / BY); }
 
-template <int P, int C, class R>
+// With NI (non-isothermal), tb is temp_bottom of this step and tproxy tells
+// whether the temperature ghost rows hold their boundary copies (first step
+// after the copies) or the linear profile.
+template <int P, int C, class R, bool NI = false>
 __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
-                                                   const R* __restrict__ mu, R* __restrict__ dphi) {
+                                                   const R* __restrict__ mu, R* __restrict__ dphi, R tb = R(0),
+                                                   int tproxy = 0) {
   const DevParams<R>& p = *prm;
   int bx, by;
   blk_xy(bx, by);
   const int y = 2 + bx * BX + threadIdx.x;
   const int x = 2 + by * BY + threadIdx.y;
   if (x > p.rx - 3 || y > p.ry - 3) return;
-  phase_cell<P, C>(p, phi, mu, dphi, x, y);
+  phase_cell<P, C, R, NI>(p, phi, mu, dphi, x, y, NI ? temp_row(p, tb, y, tproxy != 0) : R(0));
 }
 
 // mu and phi update of the interior with shared faces.  The antitrapping flux of a face is the same
@@ -244,18 +248,20 @@ __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_phase(const DevParams<R>* 
 constexpr int KM = GP_KM;  // rows per wave
 constexpr int MW = 63;    // cells per wave along y
 
-template <int P, int C, class R>
+template <int P, int C, class R, bool NI = false>
 __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP_CONC_WAVES))) k_conc_march(const DevParams<R>* __restrict__ prm,
                                                               const R* __restrict__ phi, const R* __restrict__ mu,
                                                               const R* __restrict__ dphi, const R* __restrict__ mulo,
                                                               R* __restrict__ phi_o, R* __restrict__ mu_o,
-                                                              R* __restrict__ mulo_o) {
+                                                              R* __restrict__ mulo_o, R tb = R(0)) {
   const DevParams<R>& p = *prm;
   const int lane = threadIdx.x;
   int bx, by;
   blk_xy(bx, by);
   const int y = 2 + bx * MW + lane;  // lane 0 sits one column before the wave's cells
   const int yc = min(y, p.mesh_y + 2);
+  // temperatures of the cell row and of the row above (the sweep runs after the boundary copies)
+  const R Tc = NI ? temp_row(p, tb, yc, true) : R(0), Tr = NI ? temp_row(p, tb, yc + 1, true) : R(0);
   const bool cell = lane > 0 && y <= p.mesh_y + 2;
   const int x0 = 3 + (by * BY + threadIdx.y) * KM;
   if (x0 > p.mesh_x + 2) return;  // uniform over the wave
@@ -279,7 +285,7 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
       mb[k] = mu[idx(p, k, x0 - 1, yc)];
       mc[k] = mu[idx(p, k, x0, yc)];
     }
-    face_jat<P, C>(p, n.gXb, n.cYb, n.cYc, phb, phc, mb, mc, db, dc, true, jb);
+    face_jat<P, C, R, NI>(p, n.gXb, n.cYb, n.cYc, phb, phc, mb, mc, db, dc, true, jb, Tc, Tc);
   }
   for (int i = 0; i < KM; i++) {
     const int x = x0 + i;
@@ -315,13 +321,12 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
       ml[k] = mu[idx(p, k, x, yc - 1)];
     }
     R jf[C], jr[C], jl[C];
-    face_jat<P, C>(p, n.gXc, n.cYc, n.cYf, phc, phf, mc, mf, dc, df, true, jf);
-    face_jat<P, C>(p, n.gYc, n.cXc, n.cXr, phc, phr, mc, mr, dc, dr, yc != p.yskip, jr);
+    face_jat<P, C, R, NI>(p, n.gXc, n.cYc, n.cYf, phc, phf, mc, mf, dc, df, true, jf, Tc, Tc);
+    face_jat<P, C, R, NI>(p, n.gYc, n.cXc, n.cXr, phc, phr, mc, mr, dc, dr, yc != p.yskip, jr, Tc, Tr);
 #pragma unroll
     for (int k = 0; k < C; k++) jl[k] = __shfl_up(jr[k], 1);
     if (cell)
-      conc_apply<P, C, R, !Ops<R>::Exact>(p, v, mc, mf, mb, mr, ml, dc, jf, jb, jr, jl, mulo, phi_o, mu_o, mulo_o, x,
-                                          y);
+      conc_apply<P, C, R, !Ops<R>::Exact, NI>(p, v, mc, mf, mb, mr, ml, dc, jf, jb, jr, jl, mulo, phi_o, mu_o, mulo_o, x, y, Tc);
 #pragma unroll
     for (int k = 0; k < C; k++) jb[k] = jf[k];
 #pragma unroll
@@ -334,12 +339,12 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
 // faces of its row, the first wave also its -x face, and the -x flux of row
 // w > 0 is read from row w-1 after a barrier.  1 + 1/BY + 64/63 face
 // evaluations per cell.
-template <int P, int C, class R>
+template <int P, int C, class R, bool NI = false>
 __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP_CONC_WAVES))) k_conc_lds(const DevParams<R>* __restrict__ prm,
                                                             const R* __restrict__ phi, const R* __restrict__ mu,
                                                             const R* __restrict__ dphi, const R* __restrict__ mulo,
                                                             R* __restrict__ phi_o, R* __restrict__ mu_o,
-                                                            R* __restrict__ mulo_o) {
+                                                            R* __restrict__ mulo_o, R tb = R(0)) {
   __shared__ R jx[C][BY][BX];
   const DevParams<R>& p = *prm;
   const int lane = threadIdx.x, w = threadIdx.y;
@@ -347,6 +352,8 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
   blk_xy(bx, by);
   const int y = 2 + bx * MW + lane;  // lane 0 sits one column before the wave's cells
   const int yc = min(y, p.mesh_y + 2);
+  // temperatures of the cell row and of the row above (the sweep runs after the boundary copies)
+  const R Tc = NI ? temp_row(p, tb, yc, true) : R(0), Tr = NI ? temp_row(p, tb, yc + 1, true) : R(0);
   const int x = 3 + by * BY + w;
   const int xc = min(x, p.mesh_x + 2);       // waves past the last row compute a copy and write nothing
   const bool cell = x <= p.mesh_x + 2 && lane > 0 && y <= p.mesh_y + 2;
@@ -374,8 +381,8 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
     ml[k] = mu[idx(p, k, xc, yc - 1)];
   }
   R jf[C], jb[C], jr[C], jl[C];
-  face_jat<P, C>(p, n.gXc, n.cYc, n.cYf, phc, phf, mc, mf, dc, df, true, jf);
-  face_jat<P, C>(p, n.gYc, n.cXc, n.cXr, phc, phr, mc, mr, dc, dr, yc != p.yskip, jr);
+  face_jat<P, C, R, NI>(p, n.gXc, n.cYc, n.cYf, phc, phf, mc, mf, dc, df, true, jf, Tc, Tc);
+  face_jat<P, C, R, NI>(p, n.gYc, n.cXc, n.cXr, phc, phr, mc, mr, dc, dr, yc != p.yskip, jr, Tc, Tr);
 #pragma unroll
   for (int k = 0; k < C; k++) {
     jl[k] = __shfl_up(jr[k], 1);
@@ -388,7 +395,7 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
       phb[a] = v[a][0][1];
       db[a] = dphi[idx(p, a, xc - 1, yc)];
     }
-    face_jat<P, C>(p, n.gXb, n.cYb, n.cYc, phb, phc, mb, mc, db, dc, true, jb);
+    face_jat<P, C, R, NI>(p, n.gXb, n.cYb, n.cYc, phb, phc, mb, mc, db, dc, true, jb, Tc, Tc);
   }
   __syncthreads();
   if (w > 0) {
@@ -396,7 +403,7 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
     for (int k = 0; k < C; k++) jb[k] = jx[k][w - 1][lane];
   }
   if (cell)
-    conc_apply<P, C, R, !Ops<R>::Exact>(p, v, mc, mf, mb, mr, ml, dc, jf, jb, jr, jl, mulo, phi_o, mu_o, mulo_o, x, y);
+    conc_apply<P, C, R, !Ops<R>::Exact, NI>(p, v, mc, mf, mb, mr, ml, dc, jf, jb, jr, jl, mulo, phi_o, mu_o, mulo_o, x, y, Tc);
 }
 
 // One two-pass step on `stream`: delta-phi of (phi, mu) into dphi, then the
@@ -405,20 +412,23 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
 // the barrier costs more than the face it saves (1% slower), so one row per
 // wave.  The sharing kernels beat one cell per thread also at two phases
 // (512^2: 1.3% in binary64, 5% in binary32; equal at 256^2).
-template <int P, int C, class R>
+// With NI (non-isothermal), tb is temp_bottom of the step and tproxy whether the
+// phase sweep sees the temperature boundary copies (first step after them).
+template <int P, int C, class R, bool NI = false>
 void launch_two_pass(const DevParams<R>* prm, int rx, int ry, int mesh_x, int mesh_y, hipStream_t stream,
-                     const R* phi, const R* mu, R* dphi, const R* mulo, R* phi_o, R* mu_o, R* mulo_o) {
+                     const R* phi, const R* mu, R* dphi, const R* mulo, R* phi_o, R* mu_o, R* mulo_o,
+                     R tb = R(0), int tproxy = 0) {
   const dim3 blk(BX, BY);
-  hipLaunchKernelGGL((k_phase<P, C, R>), phase_grid(rx, ry), blk, 0, stream, prm, phi, mu, dphi);
+  hipLaunchKernelGGL((k_phase<P, C, R, NI>), phase_grid(rx, ry), blk, 0, stream, prm, phi, mu, dphi, tb, tproxy);
   constexpr bool lds = Ops<R>::Exact;
   constexpr int rows = BY * (lds ? 1 : KM);
   const dim3 g((mesh_y + MW - 1) / MW, (mesh_x + rows - 1) / rows);
   if constexpr (lds)
-    hipLaunchKernelGGL((k_conc_lds<P, C, R>), g, blk, 0, stream, prm, phi, mu, (const R*)dphi, mulo, phi_o, mu_o,
-                       mulo_o);
+    hipLaunchKernelGGL((k_conc_lds<P, C, R, NI>), g, blk, 0, stream, prm, phi, mu, (const R*)dphi, mulo, phi_o, mu_o,
+                       mulo_o, tb);
   else
-    hipLaunchKernelGGL((k_conc_march<P, C, R>), g, blk, 0, stream, prm, phi, mu, (const R*)dphi, mulo, phi_o, mu_o,
-                       mulo_o);
+    hipLaunchKernelGGL((k_conc_march<P, C, R, NI>), g, blk, 0, stream, prm, phi, mu, (const R*)dphi, mulo, phi_o,
+                       mu_o, mulo_o, tb);
 }
 
 // Fill the ghost layers of a freshly loaded state: every interior cell copies
@@ -467,7 +477,8 @@ __global__ void k_unpack(const DevParams<R>* __restrict__ prm, const R* __restri
 template <int P, int C, class R>
 __global__ void __launch_bounds__(256) k_stats(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                const R* __restrict__ mu, const R* __restrict__ phi_prev,
-                                               const R* __restrict__ mu_prev, double* __restrict__ part) {
+                                               const R* __restrict__ mu_prev, double* __restrict__ part,
+                                               double tb) {
   const DevParams<R>& p = *prm;
   constexpr int NV = 3 * P + 3 * C + 1 + P + C;
   double acc[NV];
@@ -515,7 +526,12 @@ __global__ void __launch_bounds__(256) k_stats(const DevParams<R>* __restrict__ 
       for (int a = 0; a < P; a++) {
         double ca = 0.0;
 #pragma unroll
-        for (int l = 0; l < C; l++) ca += (double)p.cmu[a][k][l] * ((double)m[l] - (double)p.Bsh[a][l]);
+        for (int l = 0; l < C; l++) {
+          // non-isothermal: Beq + dBbdT (T - Teq) at the row's temperature
+          const double bs = p.ni ? (double)p.Beq[a][l] + (double)p.dBbdT[a][l] * ((tb + (double)y * (double)p.tgG) - (double)p.Teq)
+                                 : (double)p.Bsh[a][l];
+          ca += (double)p.cmu[a][k][l] * ((double)m[l] - bs);
+        }
         double pa = ph[a];
         double s1 = 0.0;
 #pragma unroll

@@ -205,6 +205,17 @@ int run_sparse(gp_ctx *c, long nsteps) {
   return GP_OK;
 }
 
+// temp_bottom after t steps of a non-isothermal run, in binary64 as the
+// reference forms it (microsim_gp.c:78-80,109-111; no moving frame, so
+// shift_OFFSET = 0)
+inline double temp_bottom(const gp_params &h, long t) {
+  const double G = (h.tempgrad[1]) * h.dy / (h.tempgrad[2]);
+  const long shift_OFFSET = 0;
+  double base = (h.tempgrad[3] / h.dy) - shift_OFFSET;
+  if (t > 0) base = (h.tempgrad[3] / h.dy) - shift_OFFSET + ((h.tempgrad[4] / h.dy) * (t * h.dt));
+  return h.tempgrad[0] - base * G;
+}
+
 template <int P, int C, class R>
 int run_steps(gp_ctx *c, long nsteps) {
   if (c->path == 0 || c->path == 2 || c->path == 3) {
@@ -222,10 +233,20 @@ int run_steps(gp_ctx *c, long nsteps) {
   }
   const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
   for (long t = 0; t < nsteps; t++) {
-    launch_two_pass<P, C, R>(prm, c->rx, c->ry, (int)c->hp.mesh_x, (int)c->hp.mesh_y, c->stream,
-                             (const R *)c->phi[c->cur], (const R *)c->mu[c->cur], (R *)c->dphi,
-                             (const R *)c->mulo[c->cur], (R *)c->phi[c->cur ^ 1], (R *)c->mu[c->cur ^ 1],
-                             (R *)c->mulo[c->cur ^ 1]);
+    if (c->hp.isothermal) {
+      launch_two_pass<P, C, R>(prm, c->rx, c->ry, (int)c->hp.mesh_x, (int)c->hp.mesh_y, c->stream,
+                               (const R *)c->phi[c->cur], (const R *)c->mu[c->cur], (R *)c->dphi,
+                               (const R *)c->mulo[c->cur], (R *)c->phi[c->cur ^ 1], (R *)c->mu[c->cur ^ 1],
+                               (R *)c->mulo[c->cur ^ 1]);
+    } else {
+      // step k = tstep + t + 1 sees the temperature set after step k-1; its phase
+      // sweep sees the temperature boundary copies only in the first step
+      const long k = c->tstep + t;
+      launch_two_pass<P, C, R, true>(prm, c->rx, c->ry, (int)c->hp.mesh_x, (int)c->hp.mesh_y, c->stream,
+                                     (const R *)c->phi[c->cur], (const R *)c->mu[c->cur], (R *)c->dphi,
+                                     (const R *)c->mulo[c->cur], (R *)c->phi[c->cur ^ 1], (R *)c->mu[c->cur ^ 1],
+                                     (R *)c->mulo[c->cur ^ 1], (R)temp_bottom(c->hp, k), k == 0 ? 1 : 0);
+    }
     c->cur ^= 1;
   }
   c->flags_valid = 0;
@@ -256,7 +277,8 @@ int run_stats(gp_ctx *c, gp_stats *st) {
   const R *pp = c->stepped ? (const R *)c->phi[c->cur ^ 1] : nullptr;
   const R *mp = c->stepped ? (const R *)c->mu[c->cur ^ 1] : nullptr;
   hipLaunchKernelGGL((k_stats<P, C, R>), dim3(c->nstat_blocks), dim3(256), 0, c->stream, prm,
-                     (const R *)c->phi[c->cur], (const R *)c->mu[c->cur], pp, mp, c->dpart);
+                     (const R *)c->phi[c->cur], (const R *)c->mu[c->cur], pp, mp, c->dpart,
+                     c->hp.isothermal ? 0.0 : temp_bottom(c->hp, c->tstep));
   GP_HIP_CHECK(hipGetLastError());
   std::vector<double> h((size_t)c->nstat_blocks * NV);
   GP_HIP_CHECK(hipMemcpyAsync(h.data(), c->dpart, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));

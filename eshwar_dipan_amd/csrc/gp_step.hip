@@ -109,6 +109,33 @@ static void fill_devparams(const gp_params &h, int rx, int ry, int pitch, long p
     }
     d.Cf[a] = (R)h.Cf[a];
   }
+  // non-isothermal runs
+  d.ni = !h.isothermal;
+  d.Teq = (R)h.Teq;
+  d.tgG = (R)(h.isothermal ? 0.0 : (h.tempgrad[1]) * h.dy / (h.tempgrad[2]));
+  d.tgv = (R)h.tempgrad[4];
+  for (int a = 0; a < P; a++)
+    for (int k = 0; k < C; k++) {
+      d.Beq[a][k] = (R)h.Beq[a][k];
+      d.dBbdT[a][k] = (R)h.dBbdT[a][k];
+      d.dC[a][k] = (R)h.delta_C[a][k];
+      d.ceqL[a][k] = (R)h.ceq[a][L][k];
+      d.ceqS[a][k] = (R)h.ceq[a][a][k];
+    }
+  for (int a = 0; a < P; a++) {
+    d.dTs[a] = (R)h.delta_T_s[a];
+    d.dTl[a] = (R)h.delta_T_l[a];
+  }
+  // temperature ghost rows (y faces): rows 0,1,2 copy proxies 2,1,0 of Y-, rows
+  // ry-3.. copy proxies 0,1,2 of Y+ (boundary_serial.h:120-175); other types copy nothing
+  for (int j = 0; j < 6; j++) d.tsrc[j] = -1;
+  {
+    const int tl = h.bc_type_T[GP_FACE_YMIN], th = h.bc_type_T[GP_FACE_YMAX];
+    if (tl == GP_BC_NEUMANN || tl == GP_BC_PERIODIC)
+      for (int j = 0; j < 3; j++) d.tsrc[j] = (int)h.bc_proxy_T[GP_FACE_YMIN][2 - j];
+    if (th == GP_BC_NEUMANN || th == GP_BC_PERIODIC)
+      for (int j = 0; j < 3; j++) d.tsrc[3 + j] = (int)h.bc_proxy_T[GP_FACE_YMAX][j];
+  }
 }
 
 template <class R, class S>
@@ -123,6 +150,7 @@ static int load_state(gp_ctx *c, const S *dphi_src, const S *dmu_src) {
   GP_HIP_CHECK(hipGetLastError());
   c->flags_valid = 0;
   c->stepped = 0;
+  c->tstep = 0;
   return GP_OK;
 }
 
@@ -248,8 +276,8 @@ static int ctx_create(const gp_params *p, int dtype, bool own_stream, gp_ctx **o
   }
   c->stream = c->own_stream;
   c->cur = 0;
-  c->path = 0;
-  c->use_graphs = 1;
+  c->path = p->isothermal ? 0 : 1;        // non-isothermal runs step on the two-pass path only
+  c->use_graphs = p->isothermal ? 1 : 0;  // and take per-step arguments (no graphs)
   c->graph_path = -1;
   c->shape = 0;
   c->shape_set = -1;
@@ -275,12 +303,18 @@ int gp_set_stream(gp_ctx *c, void *s) {
 
 int gp_set_graphs(gp_ctx *c, int on) {
   if (!c) return GP_E_ARG;
-  c->use_graphs = on != 0;
+  c->use_graphs = on != 0 && c->hp.isothermal;  // non-isothermal steps take per-step arguments
   return GP_OK;
 }
 
 int gp_set_path(gp_ctx *c, int path) {
   if (!c || path < -1 || path > 3 || (c->nmember && path != 3 && path != -1)) return GP_E_ARG;
+  if (!c->hp.isothermal) {  // non-isothermal runs step on the two-pass path only
+    if (path != 1 && path != -1) return GP_E_UNSUPPORTED;
+    c->auto_mode = 0;
+    c->path = 1;
+    return GP_OK;
+  }
   c->auto_mode = path < 0;
   c->auto_count = 0;
   c->auto_next = 0;
@@ -463,6 +497,7 @@ int gp_step_impl(gp_ctx *c, long nsteps, bool timed) {
   if (!rc && left > 0) rc = dispatch_steps(c, left);
   if (rc) return rc;
   if (nsteps > 0) c->stepped = 1;
+  c->tstep += nsteps;
   for (int i = 0; i < c->nmember; i++) {
     c->member[i]->cur = c->cur;
     c->member[i]->stepped = c->stepped;
@@ -535,6 +570,7 @@ static int autotune(gp_ctx *c, long steps, bool paths) {
 
 int gp_autotune(gp_ctx *c, long steps, int *path, int *shape) {
   if (!c || steps < 1 || c->nmember) return GP_E_ARG;
+  if (!c->hp.isothermal) return GP_E_UNSUPPORTED;
   int rc = autotune(c, steps, true);
   if (rc) return rc;
   if (path) *path = c->path;
@@ -544,6 +580,7 @@ int gp_autotune(gp_ctx *c, long steps, int *path, int *shape) {
 
 int gp_smooth(gp_ctx *c, long nsweeps) {
   if (!c || nsweeps < 0 || c->nmember) return GP_E_ARG;
+  if (!c->hp.isothermal && nsweeps > 0) return GP_E_UNSUPPORTED;
   int rc = GP_E_UNSUPPORTED;
 #define GP_DISPATCH_SMOOTH(P_, C_)                                                                  \
   if (c->P == P_ && c->C == C_)                                                                     \
@@ -617,6 +654,8 @@ int gp_ensemble_create(const gp_params *params, int n, int dtype, gp_ensemble **
   for (int i = 1; i < n; i++)
     if (params[i].nphases != params[0].nphases || params[i].ncomp != params[0].ncomp) return GP_E_ARG;
   if (!pc_supported(params[0].nphases, params[0].ncomp)) return GP_E_UNSUPPORTED;
+  for (int i = 0; i < n; i++)
+    if (!params[i].isothermal) return GP_E_UNSUPPORTED;
   gp_ensemble *e = new (std::nothrow) gp_ensemble();
   if (!e) return GP_E_NOMEM;
   gp_ctx *m = e->m = new (std::nothrow) gp_ctx();

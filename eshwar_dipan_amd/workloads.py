@@ -29,10 +29,14 @@ from . import gpstep, synth
 
 
 def params(P, C, mesh, aniso=0.0, W=1, bc=(1, 1, 1, 1), solver=None, gabc=None, rot=None,
-           dt=None, eps=None, D_sol=0.0, A_off=0.0, T=None, gamma=None, tau=None, extra=()):
+           dt=None, eps=None, D_sol=0.0, A_off=0.0, T=None, gamma=None, tau=None, extra=(), tempgrady=None):
     """GpInput for a synthetic configuration, written as Input.in text and read
-    back through gp_read_input + gp_derive."""
+    back through gp_read_input + gp_derive.  tempgrady: Tempgrady value string of
+    a non-isothermal run (ISOTHERMAL = 0, temperature copies like bc)."""
     keys = synth.base_params(P, C, mesh)
+    if tempgrady:
+        keys["ISOTHERMAL"] = "0"
+        keys["Tempgrady"] = tempgrady
     npair = P * (P - 1) // 2
     if aniso:
         keys["Function_anisotropy"] = "1"
@@ -54,6 +58,8 @@ def params(P, C, mesh, aniso=0.0, W=1, bc=(1, 1, 1, 1), solver=None, gabc=None, 
     solver = solver or ("BINARY" if C == 1 else "TERNARY" if C == 2 else "GENERAL")
     lines = synth.thermo_lines(P, C, solver=solver, D_sol=D_sol, A_off=A_off)
     lines += synth.bc_lines(bc, bc)
+    if tempgrady:
+        lines += ["BOUNDARY = {T, %d, %d, %d, %d, 0, 0};" % tuple(bc), "BOUNDARY_VALUE = {T, 0, 0, 0, 0, 0, 0};"]
     if aniso:
         lines += synth.rotation_lines(P, rot)
     lines += list(extra)

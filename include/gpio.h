@@ -45,6 +45,7 @@ typedef struct gp_input {
   int    have_gamma, have_tau, have_dab, have_gamma_abc;
   int    rot_given[GP_MAX_PHASES][GP_MAX_PHASES];
   int    bc_proxy_set[2][4];      /* BOUNDARY_VALUE assigned proxies for field/face      */
+  int    bc_proxy_set_T[4];       /* the same for the temperature                         */
   int    ncomponents_names, nphases_names;
   char   phases[GP_MAX_PHASES][GP_NAME_LEN];       /* PHASES (leading blanks kept)   */
   char   components[GP_MAX_COMP + 1][GP_NAME_LEN]; /* COMPONENTS                     */

@@ -81,7 +81,7 @@ typedef struct gp_params {
   int    obstacle;                /* 1: obstacle potential (Function_W=1), 0: well (Function_W=2) */
   int    anisotropy;              /* Function_anisotropy: 0 isotropic, 1 four-fold cubic */
   int    mu_solver;               /* GP_MU_*                                      */
-  int    isothermal;              /* ISOTHERMAL (this build steps isothermal runs) */
+  int    isothermal;              /* ISOTHERMAL (0: temperature gradient in y, below) */
   double epsilon;                 /* interface width parameter                    */
   double tau;                     /* scalar relaxation constant (functionTau.h:13) */
   double T, Teq;                  /* temperature, equilibrium temperature         */
@@ -108,6 +108,16 @@ typedef struct gp_params {
   double dBbdT[GP_MAX_PHASES][GP_MAX_COMP];
   double B[GP_MAX_PHASES][GP_MAX_COMP];                           /* B(T)         */
   double Cf[GP_MAX_PHASES];                                       /* C(T)         */
+  /* derived by gp_derive(): DELTA_T[a][a], DELTA_T[a][liquid] and DELTA_C[a] of
+     each solid (functionF_01.h:195-205), for B(T) and C(T) of other temperatures */
+  double delta_T_s[GP_MAX_PHASES], delta_T_l[GP_MAX_PHASES];
+  double delta_C[GP_MAX_PHASES][GP_MAX_COMP];
+  /* non-isothermal runs (isothermal = 0): the temperature is linear in y,
+     temp_bottom + y*DeltaT*dy/Distance, and moves with the velocity
+     (functions/Temperature_gradient.h:4-29, microsim_gp.c:77-113) */
+  double tempgrad[5];             /* Tempgrady = {base_temp, DeltaT, Distance, gradient_OFFSET, velocity} */
+  int    bc_type_T[4];            /* BOUNDARY = {T, ...} types; 0: the ghost layers are not copied */
+  long   bc_proxy_T[4][3];        /* padded index copied into each temperature ghost layer */
 } gp_params;
 
 /* Global diagnostics of the current state (replaces the TRACK_PROGRESS print of

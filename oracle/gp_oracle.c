@@ -46,35 +46,93 @@ typedef struct grid {
   long rx, ry;
   size_t n;
   int P, C;
+  /* non-isothermal runs: temp_bottom of the current step and whether the
+     temperature ghost rows hold their copies (1) or the linear profile (0) */
+  double tb;
+  int tproxy;
 } grid;
 
 #define AT(g, arr, a, x, y) (arr)[((size_t)(a) * (g)->rx + (x)) * (g)->ry + (y)]
 
 /* ---------------- thermodynamics (functionF_01.h, functionH.h) ------------ */
 
-static double f_c_mu(const gp_params *p, const double *mu, int a, int i) {
+static double f_c_mu(const gp_params *p, const double *mu, double T, int a, int i) {
   double s = 0.0;
   for (int k = 0; k < p->ncomp; k++)
-    s += p->cmu[a][i][k] * (mu[k] - (p->Beq[a][k] + p->dBbdT[a][k] * (p->T - p->Teq)));
+    s += p->cmu[a][i][k] * (mu[k] - (p->Beq[a][k] + p->dBbdT[a][k] * (T - p->Teq)));
   return s;
 }
 
-static double f_Mu(const gp_params *p, const double *c, int a, int i) {
-  double s = 2.0 * p->A[a][i][i] * c[i] + (p->Beq[a][i] + p->dBbdT[a][i] * (p->T - p->Teq));
+static double f_Mu(const gp_params *p, const double *c, double T, int a, int i) {
+  double s = 2.0 * p->A[a][i][i] * c[i] + (p->Beq[a][i] + p->dBbdT[a][i] * (T - p->Teq));
   for (int j = 0; j < p->ncomp; j++)
     if (i != j) s += p->A[a][i][j] * c[j];
   return s;
 }
 
-static double f_free_energy(const gp_params *p, const double *c, int a) {
+/* B_a,i(T) and C_a(T) (functionF_01.h:62-104) */
+static double f_B(const gp_params *p, double T, int i, int a) {
+  int C = p->ncomp, L = p->nphases - 1;
+  double cl[CM], cs[CM], sum_c = 0.0;
+  if (a == L) return 0.0;
+  for (int k = 0; k < C; k++) {
+    cl[k] = p->ceq[a][L][k] - (p->delta_C[a][k]) * (p->Teq - T) / (p->delta_T_l[a]);
+    cs[k] = p->ceq[a][a][k] - (p->delta_C[a][k]) * (p->Teq - T) / (p->delta_T_s[a]);
+    if (k != i) sum_c += p->A[L][k][i] * cl[k] - p->A[a][k][i] * cs[k];
+  }
+  return (2.0 * (p->A[L][i][i] * cl[i] - p->A[a][i][i] * cs[i]) + sum_c);
+}
+
+static double f_C(const gp_params *p, double T, int a) {
+  int C = p->ncomp, L = p->nphases - 1;
+  double cl[CM], cs[CM], sum_c = 0.0;
+  if (a == L) return 0.0;
+  for (int k = 0; k < C; k++) {
+    cl[k] = p->ceq[a][L][k] - (p->delta_C[a][k]) * (p->Teq - T) / (p->delta_T_l[a]);
+    cs[k] = p->ceq[a][a][k] - (p->delta_C[a][k]) * (p->Teq - T) / (p->delta_T_s[a]);
+  }
+  for (int i = 0; i < C; i++)
+    for (int j = 0; j < C; j++)
+      if (i <= j) sum_c += (p->A[a][i][j] * cs[i] * cs[j] - p->A[L][i][j] * cl[i] * cl[j]);
+  return sum_c;
+}
+
+/* free energy with the B and C of temperature T: the tables of the run's T when
+   isothermal, else B(T), C(T) as init_propertymatrices(T) leaves them before
+   the phase-field terms of a cell (calculate_divergence_phasefield.h:27-29) */
+static double f_free_energy(const gp_params *p, const double *c, double T, int a) {
   double s = 0.0;
   for (int i = 0; i < p->ncomp; i++) {
     for (int j = 0; j < p->ncomp; j++)
       if (i <= j) s += p->A[a][i][j] * c[i] * c[j];
-    s += p->B[a][i] * c[i];
+    s += (p->isothermal ? p->B[a][i] : f_B(p, T, i, a)) * c[i];
   }
-  s += p->Cf[a];
+  s += p->isothermal ? p->Cf[a] : f_C(p, T, a);
   return s;
+}
+
+/* temperature of padded row y (functions/Temperature_gradient.h:4-29): linear
+   in y, except ghost rows that hold the copies of the temperature boundary
+   condition (boundary_serial.h:176-203) when tproxy is set */
+static double temp_y(const grid *g, long y, int proxy) {
+  const gp_params *p = g->p;
+  long Y = y;
+  if (proxy) {
+    int tl = p->bc_type_T[GP_FACE_YMIN], th = p->bc_type_T[GP_FACE_YMAX];
+    if (y <= 2 && (tl == GP_BC_NEUMANN || tl == GP_BC_PERIODIC)) Y = p->bc_proxy_T[GP_FACE_YMIN][2 - y];
+    if (y >= g->ry - 3 && (th == GP_BC_NEUMANN || th == GP_BC_PERIODIC)) Y = p->bc_proxy_T[GP_FACE_YMAX][y - (g->ry - 3)];
+  }
+  const double G = (p->tempgrad[1]) * p->dy / (p->tempgrad[2]);
+  return g->tb + Y * G;
+}
+
+/* temp_bottom after t steps (microsim_gp.c:78-80,109-111; no moving frame) */
+static double temp_bottom(const gp_params *p, long t) {
+  const double G = (p->tempgrad[1]) * p->dy / (p->tempgrad[2]);
+  const long shift_OFFSET = 0;
+  double base = (p->tempgrad[3] / p->dy) - shift_OFFSET;
+  if (t > 0) base = (p->tempgrad[3] / p->dy) - shift_OFFSET + ((p->tempgrad[4] / p->dy) * (t * p->dt));
+  return p->tempgrad[0] - base * G;
 }
 
 /* dh_b / dphi_a */
@@ -103,15 +161,15 @@ static double f_hphi(const double *phi, int P, int a) {
   return s + s1;
 }
 
-static double f_dpsi(const gp_params *p, const double *mu, const double *phi, int a) {
+static double f_dpsi(const gp_params *p, const double *mu, const double *phi, double T, int a) {
   double sum = 0.0;
   for (int b = 0; b < p->nphases; b++) {
     double c[CM], psi = 0.0;
     for (int k = 0; k < p->ncomp; k++) {
-      c[k] = f_c_mu(p, mu, b, k);
+      c[k] = f_c_mu(p, mu, T, b, k);
       psi -= mu[k] * c[k];
     }
-    psi += f_free_energy(p, c, b);
+    psi += f_free_energy(p, c, T, b);
     sum += psi * f_dhphi(phi, p->nphases, b, a);
   }
   return sum;
@@ -375,6 +433,7 @@ static void stage_dphi(const grid *g, const double *phi, const double *mu, work 
   for (long x = 2; x <= g->rx - 3; x++)
     for (long y = 2; y <= g->ry - 3; y++) {
       double ph[PM], m[CM], divphi[PM], lam[PM], dp[PM];
+      const double T = p->isothermal ? p->T : temp_y(g, y, g->tproxy);
       load_cell(g, phi, mu, x, y, ph, m);
       for (int a = 0; a < P; a++) {
         divphi[a] = (AT(g, w->gX, a, x, y) - AT(g, w->gX, a, x - 1, y)) / p->dx;
@@ -385,7 +444,7 @@ static void stage_dphi(const grid *g, const double *phi, const double *mu, work 
       for (int a = 0; a < P; a++)
         if (fabs(divphi[a]) > 0.0) {
           lam[a] = p->epsilon * (-dAdphi(g, w, ph, x, y, a) + divdAdgradphi(g, w, x, y, a)) -
-                   (1.0 / p->epsilon) * dwdphi(p, ph, divphi, a) - f_dpsi(p, m, ph, a);
+                   (1.0 / p->epsilon) * dwdphi(p, ph, divphi, a) - f_dpsi(p, m, ph, T, a);
           sl += lam[a];
           active++;
         }
@@ -452,6 +511,9 @@ static void stage_fluxes(const grid *g, const double *phi, const double *mu, wor
       load_cell(g, phi, mu, x, y, ph, m);
       load_cell(g, phi, mu, x + 1, y, phf, mf);
       load_cell(g, phi, mu, x, y + 1, phr, mr);
+      /* the concentration sweep runs after the boundary copies: ghost rows hold copies */
+      const double Tc = p->isothermal ? p->T : temp_y(g, y, 1);
+      const double Tr = p->isothermal ? p->T : temp_y(g, y + 1, 1);
       double jX[CM], jY[CM];
       double gxl = 0.5 * (AT(g, w->cX, L, x, y) + AT(g, w->cX, L, x, y + 1));
       double ngyl = sqrt(gxl * gxl + AT(g, w->gY, L, x, y) * AT(g, w->gY, L, x, y));
@@ -469,8 +531,8 @@ static void stage_fluxes(const grid *g, const double *phi, const double *mu, wor
         }
         for (int k = 0; k < C; k++) {
           if (ngp != 0.0 && y != 0 && y != (p->mesh_y - 2)) {
-            double clr = f_c_mu(p, mr, L, k), clc = f_c_mu(p, m, L, k);
-            double csr = f_c_mu(p, mr, a, k), csc = f_c_mu(p, m, a, k);
+            double clr = f_c_mu(p, mr, Tr, L, k), clc = f_c_mu(p, m, Tc, L, k);
+            double csr = f_c_mu(p, mr, Tr, a, k), csc = f_c_mu(p, m, Tc, a, k);
             double sp = AT(g, w->gY, a, x, y) * AT(g, w->gY, L, x, y);
             sp += gx * gxl;
             if (ngyl > 0.0) sp /= (ngyl * ngp);
@@ -485,8 +547,8 @@ static void stage_fluxes(const grid *g, const double *phi, const double *mu, wor
         ngp = sqrt(gy * gy + AT(g, w->gX, a, x, y) * AT(g, w->gX, a, x, y));
         for (int k = 0; k < C; k++) {
           if (ngp != 0.0) {
-            double clf = f_c_mu(p, mf, L, k), clc = f_c_mu(p, m, L, k);
-            double csf = f_c_mu(p, mf, a, k), csc = f_c_mu(p, m, a, k);
+            double clf = f_c_mu(p, mf, Tc, L, k), clc = f_c_mu(p, m, Tc, L, k);
+            double csf = f_c_mu(p, mf, Tc, a, k), csc = f_c_mu(p, m, Tc, a, k);
             double sp = AT(g, w->gX, a, x, y) * AT(g, w->gX, L, x, y);
             sp += gy * gyl;
             if (ngxl > 0.0) sp /= (ngxl * ngp);
@@ -516,6 +578,7 @@ static void stage_update(const grid *g, const double *phi, const double *mu, wor
     for (long y = 3; y <= MY + 2; y++) {
       double ph[PM], m[CM], dp[PM], sum[CM], divflux[CM], divjat[CM], dmu[CM];
       double dcdmu[CM][CM];
+      const double T = p->isothermal ? p->T : temp_y(g, y, 1);
       load_cell(g, phi, mu, x, y, ph, m);
       for (int a = 0; a < P; a++) dp[a] = AT(g, w->dphi, a, x, y);
       for (int k = 0; k < C; k++) {
@@ -525,7 +588,7 @@ static void stage_update(const grid *g, const double *phi, const double *mu, wor
       for (int a = 0; a < P; a++) {
         double sdh = 0.0;
         for (int b = 0; b < P; b++) sdh += f_dhphi(ph, P, a, b) * dp[b];
-        for (int k = 0; k < C; k++) sum[k] += f_c_mu(p, m, a, k) * sdh;
+        for (int k = 0; k < C; k++) sum[k] += f_c_mu(p, m, T, a, k) * sdh;
       }
       for (int k = 0; k < C; k++) {
         divflux[k] = 0.0;
@@ -541,12 +604,22 @@ static void stage_update(const grid *g, const double *phi, const double *mu, wor
         if (ph[a] == 1.0) { bulk = a; break; }
       if (bulk >= 0) {
         double cn[CM];
-        for (int k = 0; k < C; k++) cn[k] = f_c_mu(p, m, bulk, k) + p->dt * (divflux[k]) - divjat[k];
-        for (int k = 0; k < C; k++) AT(g, mu_o, k, x, y) = f_Mu(p, cn, bulk, k);
+        for (int k = 0; k < C; k++) cn[k] = f_c_mu(p, m, T, bulk, k) + p->dt * (divflux[k]) - divjat[k];
+        for (int k = 0; k < C; k++) AT(g, mu_o, k, x, y) = f_Mu(p, cn, T, bulk, k);
       } else {
         double dc[CM];
         for (int k = 0; k < C; k++) {
-          dc[k] = p->dt * (divflux[k]) - divjat[k] - sum[k];
+          if (p->isothermal) {
+            dc[k] = p->dt * (divflux[k]) - divjat[k] - sum[k];
+          } else {
+            /* dcbdT_phase is allocated zero and never set, nor is the GRADIENT
+               member of temperature_gradientY, so the term is a signed zero
+               (calculate_divergence_concentration.h:22-40,85) */
+            double sum_dcbdT = 0.0;
+            for (int a = 0; a < P; a++) sum_dcbdT += 0.0 * f_hphi(ph, P, a);
+            const double grad_member = 0.0;
+            dc[k] = p->dt * (divflux[k]) - divjat[k] - sum[k] - sum_dcbdT * p->dt * (-grad_member * p->tempgrad[4]);
+          }
           for (int l = 0; l < C; l++)
             for (int a = 0; a < P; a++) dcdmu[k][l] += p->cmu[a][k][l] * f_hphi(ph, P, a);
         }
@@ -700,8 +773,8 @@ static void stage_smooth(const grid *g, const double *phi, const double *mu, wor
         if (ph[a] == 1.0) { bulk = a; break; }
       if (bulk >= 0) {
         double cn[CM];
-        for (int k = 0; k < C; k++) cn[k] = f_c_mu(p, m, bulk, k) + p->dt * (divflux[k]);
-        for (int k = 0; k < C; k++) AT(g, mu_o, k, x, y) = f_Mu(p, cn, bulk, k);
+        for (int k = 0; k < C; k++) cn[k] = f_c_mu(p, m, p->T, bulk, k) + p->dt * (divflux[k]);
+        for (int k = 0; k < C; k++) AT(g, mu_o, k, x, y) = f_Mu(p, cn, p->T, bulk, k);
       } else {
         double dc[CM], a16[16], i16[16];
         for (int k = 0; k < C; k++)
@@ -828,6 +901,8 @@ int gpo_dphi(const gp_params *p, const double *phi, const double *mu, double *dp
   if (rc) return rc;
   grid G = {p, p->mesh_x + 6, p->mesh_y + 6, 0, p->nphases, p->ncomp};
   G.n = (size_t)G.rx * G.ry;
+  G.tb = temp_bottom(p, 0);
+  G.tproxy = 1;
   work w;
   if (alloc_work(&G, &w)) { free_work(&w); return GP_E_NOMEM; }
   stage_gradients(&G, phi, &w);
@@ -838,6 +913,9 @@ int gpo_dphi(const gp_params *p, const double *phi, const double *mu, double *dp
 }
 
 int gpo_step(const gp_params *p, double *phi, double *mu, long nsteps, int nthreads) {
+  return gpo_step_from(p, phi, mu, 0, nsteps, nthreads);
+}
+int gpo_step_from(const gp_params *p, double *phi, double *mu, long t0, long nsteps, int nthreads) {
   int rc = check(p);
   if (rc) return rc;
 #ifdef _OPENMP
@@ -855,6 +933,10 @@ int gpo_step(const gp_params *p, double *phi, double *mu, long nsteps, int nthre
     return GP_E_NOMEM;
   }
   for (long t = 0; t < nsteps; t++) {
+    /* step t+1 sees the temperature set after step t; its phase sweep sees the
+       ghost copies only in the first step (after the initial boundary copies) */
+    G.tb = temp_bottom(p, t0 + t);
+    G.tproxy = t0 + t == 0;
     memcpy(phi_o, phi, sizeof(double) * G.n * G.P);
     memcpy(mu_o, mu, sizeof(double) * G.n * G.C);
     stage_gradients(&G, phi, &w);
@@ -878,6 +960,7 @@ int gpo_step(const gp_params *p, double *phi, double *mu, long nsteps, int nthre
 int gpo_smooth(const gp_params *p, double *phi, double *mu, long nsweeps, int nthreads) {
   int rc = check(p);
   if (rc) return rc;
+  if (!p->isothermal) return GP_E_UNSUPPORTED;
 #ifdef _OPENMP
   int saved = omp_get_max_threads();
   if (nthreads > 0) omp_set_num_threads(nthreads);

@@ -23,15 +23,21 @@ extern "C" {
 int gpo_apply_bc(const gp_params *p, double *phi, double *mu);
 
 /* nsteps full steps in place (microsim_gp.c:103-106); ghosts are refreshed after
-   each step.  nthreads <= 0 uses OpenMP's default, 1 runs serially. */
+   each step.  nthreads <= 0 uses OpenMP's default, 1 runs serially.  For a
+   non-isothermal p the call starts at time step 0 right after the boundary
+   copies, as the reference's time loop does (microsim_gp.c:77-113). */
 int gpo_step(const gp_params *p, double *phi, double *mu, long nsteps, int nthreads);
+/* The same, continuing a run after t0 steps (the temperature of a
+   non-isothermal run depends on the time step). */
+int gpo_step_from(const gp_params *p, double *phi, double *mu, long t0, long nsteps, int nthreads);
 
 /* Delta-phi of one phase sweep on the ring-extended region x,y in [2, rows-3]
    (zero elsewhere), for tests (calculate_divergence_phasefield.h:4-62). */
 int gpo_dphi(const gp_params *p, const double *phi, const double *mu, double *dphi);
 
 /* nsweeps pre-loop smoothing sweeps in place (microsim_gp.c:97-100, smooth()
-   of solverloop/solverloop.h:72-90), ghosts refreshed after each. */
+   of solverloop/solverloop.h:72-90), ghosts refreshed after each; isothermal
+   parameter sets only. */
 int gpo_smooth(const gp_params *p, double *phi, double *mu, long nsweeps, int nthreads);
 
 #ifdef __cplusplus

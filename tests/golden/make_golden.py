@@ -125,6 +125,12 @@ CASES = [
          fill=["FILLCYLINDER = {0, 22, 20, 0, 0, 7};"], steps=20, save=[1, 20]),
     dict(name="p3c2_smooth_ternary_seeds", P=3, C=2, mesh=(40, 40), solver="TERNARY", nsmooth=6, W=2,
          gabc=2.0, init="seeds", nseeds=4, r=(4, 8), diffuse=0.0, noise=1e-3, steps=10, save=[1, 10]),
+    # non-isothermal: linear temperature in y moving with the velocity of Tempgrady
+    dict(name="p2c1_tempgrady_fill", P=2, C=1, mesh=(56, 64), aniso=0.02, tempgrady="{0.96, 0.06, 80.0, 5, 1.6}",
+         init="fill", fill=["FILLCYLINDER = {0, 28, 20, 0, 0, 8};"], steps=200, save=[1, 100, 200]),
+    dict(name="p3c2_tempgrady_ternary_periodic", P=3, C=2, mesh=(40, 48), solver="TERNARY", bc=(1, 1, 3, 3),
+         tempgrady="{0.94, 0.1, 60.0, 2, 2.0}", init="fill",
+         fill=["FILLCYLINDER = {0, 12, 14, 0, 0, 6};", "FILLCYLINDER = {1, 28, 30, 0, 0, 7};"], steps=60, save=[1, 60]),
     dict(name="p4c3_general_cube", P=4, C=3, mesh=(32, 32), solver="GENERAL", A_off=0.2,
          init="fill", fill=["FILLCUBE = {0, 4, 4, 0, 12, 12, 0};", "FILLCYLINDER = {1, 22, 20, 0, 0, 6};",
                             "FILLSPHERE = {2, 10, 24, 0, 5};"], steps=30, save=[1, 30]),
@@ -149,6 +155,9 @@ def build_case(c):
         keys["Gamma_abc"] = "{" + ",".join([str(c["gabc"])] * (P * (P - 1) * (P - 2) // 6)) + "}"
     if c.get("tau_str"):
         keys["Tau"] = "{" + ",".join([c["tau_str"]] * npair) + "}"
+    if c.get("tempgrady"):
+        keys["ISOTHERMAL"] = "0"
+        keys["Tempgrady"] = c["tempgrady"]
     solver = c.get("solver", "BINARY")
     extra = []
     if solver == "BINARY_DILUTE0":
@@ -159,6 +168,8 @@ def build_case(c):
     extra += th
     bc = c.get("bc", (1, 1, 1, 1))
     extra += synth.bc_lines(bc, bc)
+    if c.get("tempgrady"):
+        extra += ["BOUNDARY = {T, %d, %d, %d, %d, 0, 0};" % tuple(bc), "BOUNDARY_VALUE = {T, 0, 0, 0, 0, 0, 0};"]
     if c.get("aniso"):
         extra += synth.rotation_lines(P, c.get("rot"))
     if c["init"] != "fill":

@@ -22,8 +22,9 @@ def oracle():
         if not os.path.exists(ORACLE_LIB):
             raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
         L = ct.CDLL(ORACLE_LIB)
-        for n in ("gpo_apply_bc", "gpo_step", "gpo_dphi", "gpo_smooth"):
+        for n in ("gpo_apply_bc", "gpo_step", "gpo_step_from", "gpo_dphi", "gpo_smooth"):
             getattr(L, n).restype = ct.c_int
+        L.gpo_step_from.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p, ct.c_long, ct.c_long, ct.c_int]
         L.gpo_apply_bc.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p]
         L.gpo_step.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p, ct.c_long, ct.c_int]
         L.gpo_dphi.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p, ct.c_void_p]
@@ -47,10 +48,11 @@ def pad(p, phi, mu):
     return php, mup
 
 
-def oracle_run(p, phi, mu, nsteps, nthreads=0):
-    """Advance interior fields nsteps with the oracle; returns interior copies."""
+def oracle_run(p, phi, mu, nsteps, nthreads=0, t0=0):
+    """Advance interior fields nsteps with the oracle, continuing a run after t0
+    steps; returns interior copies."""
     php, mup = pad(p, phi, mu)
-    rc = oracle().gpo_step(ct.byref(p), _p(php), _p(mup), int(nsteps), int(nthreads))
+    rc = oracle().gpo_step_from(ct.byref(p), _p(php), _p(mup), int(t0), int(nsteps), int(nthreads))
     assert rc == 0, rc
     return php[:, 3:-3, 3:-3].copy(), mup[:, 3:-3, 3:-3].copy()
 

@@ -22,6 +22,11 @@ def _cmp(a, b):
 def test_gpu_f64_reproduces_reference(name, path):
     print(f"\n[gpu] golden {name} path={path}", flush=True)
     z, inp, _ = gputil.load_golden(name)
+    if not inp.p.isothermal and path != 1:
+        # non-isothermal runs step on the two-pass path only; the others refuse
+        with pytest.raises(gpstep.GPError):
+            gpstep.Solver(inp, "f64", path=path)
+        return
     s = gpstep.Solver(inp, "f64", path=path)
     s.set_state(z["phi_0"], z["mu_0"])
     if inp.nsmooth > 1:
@@ -47,6 +52,10 @@ ORACLE_CASES = [
      dict(nseeds=1, r=(6, 6), diffuse=1.0), 300),
     ("c2_1024_p4c2_full_size", dict(P=4, C=2, mesh=(1024, 1024), aniso=0.02, bc=(3, 3, 3, 3), gabc=1.0),
      dict(nseeds=32, r=(6, 12), diffuse=0.0, noise=1e-3), 20),
+    # non-isothermal: temperature gradient in y moving with the Tempgrady velocity
+    ("ni_192_p3c2_tempgrady_periodic_y", dict(P=3, C=2, mesh=(160, 192), aniso=0.02, bc=(1, 1, 3, 3),
+                                              tempgrady="{0.95, 0.08, 120.0, 10, 2.5}"),
+     dict(nseeds=12, r=(5, 10), diffuse=1.0, noise=1e-3), 150),
 ]
 
 
@@ -69,10 +78,15 @@ def test_gpu_f64_matches_oracle(name, spec, init, steps):
     print(f"\n[gpu] oracle {name} steps={steps}", flush=True)
     inp, phi0, mu0 = _oracle_case(spec, init, seed=sum(map(ord, name)))
     ophi, omu = gputil.oracle_run(inp.p, phi0, mu0, steps, nthreads=0)
-    for path in (1, 0, 2, 3):
+    ni = not inp.p.isothermal
+    for path in ((1,) if ni else (1, 0, 2, 3)):
         s = gpstep.Solver(inp, "f64", path=path)
         s.set_state(phi0, mu0)
-        s.step(steps)
+        if ni:  # the temperature follows the step count across calls
+            s.step(steps // 3)
+            s.step(steps - steps // 3)
+        else:
+            s.step(steps)
         phi, mu = s.get_state()
         ep, bp = _cmp(phi, ophi)
         em, bm = _cmp(mu, omu)
@@ -138,6 +152,69 @@ def test_gpu_stats_and_simplex():
         np.testing.assert_allclose(st["solute"][k], float((w * tot).sum()), rtol=1e-11)
     ophi, omu = gputil.oracle_run(inp.p, phi0, mu0, 50, nthreads=0)
     assert np.abs(phi - ophi).max() <= TOL64 and np.abs(mu - omu).max() <= TOL64
+
+
+def _temp_bottom(p, t):
+    """temp_bottom after t steps (microsim_gp.c:78-80,109-111), in binary64."""
+    G = p.tempgrad[1] * p.dy / p.tempgrad[2]
+    base = p.tempgrad[3] / p.dy - 0
+    if t > 0:
+        base = p.tempgrad[3] / p.dy - 0 + ((p.tempgrad[4] / p.dy) * (t * p.dt))
+    return p.tempgrad[0] - base * G, G
+
+
+@pytest.mark.gpu
+def test_gpu_nonisothermal_f32_stats_and_limits():
+    """Non-isothermal run (Tempgrady): binary32 close to binary64; the solute sum
+    uses each row's temperature; paths other than two-pass, smoothing, ensembles
+    and the adjoint refuse such a run."""
+    print("\n[gpu] non-isothermal f32 vs f64, stats, refusals, 128x112 dendrite, 200 steps", flush=True)
+    inp = gputil.make_params(P=2, C=1, mesh=(128, 112), aniso=0.02, tempgrady="{0.95, 0.06, 100.0, 4, 2.0}")
+    p = inp.p
+    assert not p.isothermal
+    phi0, mu0 = gputil.seeded_state(inp, 1, 8, 8, 11, diffuse=1.5, noise=0.0)
+    out = {}
+    for dt in ("f64", "f32"):
+        s = gpstep.Solver(inp, dt)
+        s.set_state(phi0, mu0)
+        s.step(200)
+        out[dt] = s.get_state()
+        if dt == "f64":
+            st = s.stats()
+    ephi = np.abs(out["f32"][0] - out["f64"][0]).max()
+    emu = np.abs(out["f32"][1] - out["f64"][1]).max() / max(np.ptp(out["f64"][1]), 1e-30)
+    print(f"  max|dphi|={ephi:.3e} rel max|dmu|={emu:.3e}", flush=True)
+    assert ephi < 1e-4 and emu < 1e-4
+    phi, mu = out["f64"]
+    tb, G = _temp_bottom(p, 200)
+    T = tb + (np.arange(p.mesh_y) + 3.0) * G   # padded row index y + 3
+    P, C = p.nphases, p.ncomp
+    h = []
+    for a in range(P):
+        s1 = sum(phi[b] * phi[c] for b in range(P) for c in range(P) if b != a and c != a and b < c)
+        h.append(3 * phi[a] ** 2 - 2 * phi[a] ** 3 + 2 * phi[a] * s1)
+    w = np.ones(phi.shape[1:])
+    w[0, :] *= 0.5
+    w[-1, :] *= 0.5
+    w[:, 0] *= 0.5
+    w[:, -1] *= 0.5
+    for k in range(C):
+        tot = 0.0
+        for a in range(P):
+            ca = sum(p.cmu[a][k][l] * (mu[l] - (p.Beq[a][l] + p.dBbdT[a][l] * (T[None, :] - p.Teq))) for l in range(C))
+            tot = tot + h[a] * ca
+        np.testing.assert_allclose(st["solute"][k], float((w * tot).sum()), rtol=1e-11)
+    for path in (0, 2, 3):
+        with pytest.raises(gpstep.GPError):
+            gpstep.Solver(inp, "f64", path=path)
+    s = gpstep.Solver(inp, "f64")
+    s.set_state(phi0, mu0)
+    with pytest.raises(gpstep.GPError):
+        s.smooth(1)
+    with pytest.raises(gpstep.GPError):
+        gpstep.Adjoint(s, 10)
+    with pytest.raises(gpstep.GPError):
+        gpstep.Ensemble([inp], "f64")
 
 
 @pytest.mark.gpu

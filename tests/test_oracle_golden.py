@@ -19,7 +19,7 @@ def test_golden_matrix_is_complete():
     assert len(CASES) >= 12
     names = " ".join(CASES)
     for tag in ("aniso", "well", "periodic", "ternary", "dilute", "general", "p7", "p4c3", "mixed_bc", "fill",
-                "smooth"):
+                "smooth", "tempgrady"):
         assert tag in names, tag
 
 
@@ -32,7 +32,7 @@ def test_oracle_reproduces_reference(name):
         phi, mu = gputil.oracle_smooth(p, phi, mu, inp.nsmooth - 1, nthreads=4)
     done = 0
     for t in z["steps"]:
-        phi, mu = gputil.oracle_run(p, phi, mu, int(t) - done, nthreads=4)
+        phi, mu = gputil.oracle_run(p, phi, mu, int(t) - done, nthreads=4, t0=done)
         done = int(t)
         assert np.array_equal(phi, z[f"phi_{t}"]), f"{name}: phi differs after {t} steps, max {np.abs(phi - z[f'phi_{t}']).max()}"
         assert np.array_equal(mu, z[f"mu_{t}"]), f"{name}: mu differs after {t} steps, max {np.abs(mu - z[f'mu_{t}']).max()}"
