@@ -745,8 +745,35 @@ static double to_big_endian(double v) {
   return v;
 }
 
+/* temp_bottom after t steps of a non-isothermal run without moving frame
+   (microsim_gp.c:78-80 for t = 0, 109-111 after step t; shift_OFFSET = 0) */
+double gp_temp_bottom(const gp_params *p, long t) {
+  const double G = (p->tempgrad[1]) * p->dy / (p->tempgrad[2]);
+  const long shift_OFFSET = 0;
+  double base = (p->tempgrad[3] / p->dy) - shift_OFFSET;
+  if (t > 0) base = (p->tempgrad[3] / p->dy) - shift_OFFSET + ((p->tempgrad[4] / p->dy) * (t * p->dt));
+  return p->tempgrad[0] - base * G;
+}
+
+static void put_value(FILE *f, double v, int binary, const char *fmt) {
+  if (binary) { v = to_big_endian(v); fwrite(&v, 8, 1, f); }
+  else fprintf(f, fmt, v);
+}
+
 int gp_vtk_write(const char *path, const gp_input *in, const double *phi, const double *mu, int binary) {
+  return gp_vtk_write_t(path, in, phi, mu, binary, 0);
+}
+
+/* write_cells_vtk_2D / write_cells_vtk_2D_binary (solverloop/file_writer.h:
+   60-253): phases, Mu_<k>, Composition_<k> when WRITECOMPOSITION, and for a
+   non-isothermal run the Temperature block.  The temperature of interior row
+   y (padded index) after t steps is temp_bottom + y*GRADIENT
+   (functions/Temperature_gradient.h:15-19), which the composition also uses. */
+int gp_vtk_write_t(const char *path, const gp_input *in, const double *phi, const double *mu, int binary, long t) {
   const gp_params *p = &in->p;
+  const int ni = !p->isothermal;
+  const double tb = ni ? gp_temp_bottom(p, t) : 0.0;
+  const double G = ni ? (p->tempgrad[1]) * p->dy / (p->tempgrad[2]) : 0.0;
   FILE *f = fopen(path, binary ? "wb" : "w");
   if (!f) return GP_E_IO;
   long MX = p->mesh_x, MY = p->mesh_y;
@@ -787,12 +814,17 @@ int gp_vtk_write(const char *path, const gp_input *in, const double *phi, const 
         double ph[P_MAX], m[C_MAX], comp = 0.0;
         for (int a = 0; a < p->nphases; a++) ph[a] = phi[a * n + i];
         for (int l = 0; l < p->ncomp; l++) m[l] = mu[l * n + i];
-        for (int b = 0; b < p->nphases; b++) comp += c_of_mu(p, m, p->T, b, k) * h_of_phi(ph, p->nphases, b);
-        if (binary) { double v = to_big_endian(comp); fwrite(&v, 8, 1, f); }
-        else fprintf(f, "%le \n", comp);
+        const double T = ni ? tb + (double)(i % MY + 3) * G : p->T;
+        for (int b = 0; b < p->nphases; b++) comp += c_of_mu(p, m, T, b, k) * h_of_phi(ph, p->nphases, b);
+        put_value(f, comp, binary, "%le \n");
       }
       if (!binary) fprintf(f, "\n");
     }
+  }
+  if (ni) {
+    fprintf(f, "SCALARS Temperature double 1\n");
+    fprintf(f, "LOOKUP_TABLE default\n");
+    for (size_t i = 0; i < n; i++) put_value(f, tb + (double)(i % MY + 3) * G, binary, "%le \n");
   }
   int bad = ferror(f);
   fclose(f);

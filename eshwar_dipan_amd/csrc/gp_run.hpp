@@ -208,13 +208,7 @@ int run_sparse(gp_ctx *c, long nsteps) {
 // temp_bottom after t steps of a non-isothermal run, in binary64 as the
 // reference forms it (microsim_gp.c:78-80,109-111; no moving frame, so
 // shift_OFFSET = 0)
-inline double temp_bottom(const gp_params &h, long t) {
-  const double G = (h.tempgrad[1]) * h.dy / (h.tempgrad[2]);
-  const long shift_OFFSET = 0;
-  double base = (h.tempgrad[3] / h.dy) - shift_OFFSET;
-  if (t > 0) base = (h.tempgrad[3] / h.dy) - shift_OFFSET + ((h.tempgrad[4] / h.dy) * (t * h.dt));
-  return h.tempgrad[0] - base * G;
-}
+inline double temp_bottom(const gp_params &h, long t) { return gp_temp_bottom(&h, t); }
 
 template <int P, int C, class R>
 int run_steps(gp_ctx *c, long nsteps) {

@@ -12,9 +12,11 @@
  * accepted and ignored (one GPU holds the whole domain).
  *
  * Environment: GP_DTYPE=f32 selects the binary32 step (default binary64).
+ * Non-isothermal runs (ISOTHERMAL = 0, Tempgrady) step with the row
+ * temperature of the reference; their files carry the Temperature block.
  * Differences from the reference are reported instead of silently diverging:
- * runs that need a moving frame (Shift), a temperature gradient or phase-field
- * noise exit with a message and status 2.
+ * runs that need a moving frame (Shift), phase-field noise, or smoothing
+ * sweeps of a non-isothermal run exit with a message and status 2.
  */
 #include <math.h>
 #include <stdio.h>
@@ -52,8 +54,12 @@ int main(int argc, char *argv[]) {
     fprintf(stderr, "microsim_gp: %s\n", err);
     return 2;
   }
-  if (in.shift || in.tempgrady || in.noise_phasefield) {
-    fprintf(stderr, "microsim_gp: Shift, Tempgrady and Noise_phasefield runs are not stepped by this build\n");
+  if (in.shift || in.noise_phasefield) {
+    fprintf(stderr, "microsim_gp: Shift and Noise_phasefield runs are not stepped by this build\n");
+    return 2;
+  }
+  if (!in.p.isothermal && in.nsmooth > 1) {
+    fprintf(stderr, "microsim_gp: smoothing sweeps (NSMOOTH > 1) of a non-isothermal run are not stepped by this build\n");
     return 2;
   }
   if (in.saveT <= 0 || in.track_progress <= 0) {
@@ -78,6 +84,10 @@ int main(int argc, char *argv[]) {
     snprintf(name, sizeof name, "DATA/%s_%ld.vtk", argv[3], in.starttime);
     rc = gp_vtk_read(name, &in, phi, mu, !in.ascii);
     if (rc) return fail(name, rc);
+    /* a restart moves the gradient by the distance travelled up to STARTTIME
+       (microsim_gp.c:67-74, no moving frame) */
+    if (!in.p.isothermal)
+      in.p.tempgrad[3] = (in.p.tempgrad[3]) + floor((in.p.tempgrad[4]) * (in.starttime * in.p.dt));
   }
 
   gp_ctx *ctx = NULL;
@@ -124,7 +134,7 @@ int main(int argc, char *argv[]) {
       rc = gp_get_state(ctx, phi, mu);
       if (rc) return fail("reading back the state", rc);
       snprintf(name, sizeof name, "DATA/%s_%ld.vtk", argv[3], t + in.starttime);
-      rc = gp_vtk_write(name, &in, phi, mu, !in.ascii);
+      rc = gp_vtk_write_t(name, &in, phi, mu, !in.ascii, t);
       if (rc) return fail(name, rc);
       FILE *fp = fopen("DATA/shift.dat", "a");
       if (fp) {

@@ -48,6 +48,7 @@ def lib():
             ("gp_check_supported", [ct.POINTER(GpParams), ct.c_char_p, ct.c_size_t]),
             ("gp_fill_domain", [ct.POINTER(GpInput), ct.c_char_p, vp, vp, i, ct.c_char_p, ct.c_size_t]),
             ("gp_vtk_write", [ct.c_char_p, ct.POINTER(GpInput), vp, vp, i]),
+            ("gp_vtk_write_t", [ct.c_char_p, ct.POINTER(GpInput), vp, vp, i, l]),
             ("gp_vtk_read", [ct.c_char_p, ct.POINTER(GpInput), vp, vp, i]),
             ("gp_matinv", [vp, vp, i]),
             ("gp_abi_sizes", [ct.POINTER(ct.c_size_t)] * 3),
@@ -137,10 +138,19 @@ def vtk_read(path, inp, binary=True):
     return phi, mu
 
 
-def vtk_write(path, inp, phi, mu, binary=True):
+def vtk_write(path, inp, phi, mu, binary=True, step=0):
+    """Legacy VTK of the state after `step` steps (the step sets the row
+    temperature of a non-isothermal input; isothermal files do not depend on it)."""
     phi = np.ascontiguousarray(phi, dtype=np.float64)
     mu = np.ascontiguousarray(mu, dtype=np.float64)
-    _check(lib().gp_vtk_write(os.fsencode(path), ct.byref(inp), _ptr(phi), _ptr(mu), int(binary)), path)
+    _check(lib().gp_vtk_write_t(os.fsencode(path), ct.byref(inp), _ptr(phi), _ptr(mu), int(binary), int(step)), path)
+
+
+def temp_bottom(params, step):
+    """Temperature of padded row 0 of a non-isothermal run after `step` steps."""
+    f = lib().gp_temp_bottom
+    f.argtypes, f.restype = [ct.POINTER(GpParams), ct.c_long], ct.c_double
+    return f(ct.byref(params), int(step))
 
 
 class Solver:

@@ -62,8 +62,9 @@ int gp_read_input(const char *path, gp_input *in, int verbose, char *err, size_t
    init_propertymatrices(T) leaves them.  Returns GP_OK or GP_E_ARG. */
 int gp_derive(gp_params *p);
 
-/* Check that p can be stepped by this build (isothermal, F_01, copy-type
-   boundaries with interior proxies, supported P/C).  GP_OK or a code with message. */
+/* Check that p can be stepped by this build (F_01, copy-type boundaries with
+   interior proxies, supported P/C; a non-isothermal run needs Tempgrady with a
+   non-zero Distance).  GP_OK or a code with message. */
 int gp_check_supported(const gp_params *p, char *err, size_t errlen);
 
 /* Initial condition from Filling.in into interior arrays phi[P][MX][MY],
@@ -76,6 +77,12 @@ int gp_fill_domain(const gp_input *in, const char *filling_path,
    The composition block (Writecomposition) is written when in->writecomposition. */
 int gp_vtk_write(const char *path, const gp_input *in, const double *phi,
                  const double *mu, int binary);
+/* gp_vtk_write for the state after t steps of this run: for a non-isothermal
+   input the composition uses the row temperature gp_temp_bottom(p, t) +
+   y*GRADIENT and a Temperature block follows (file_writer.h:137-148, 238-252).
+   gp_vtk_write is this call with t = 0. */
+int gp_vtk_write_t(const char *path, const gp_input *in, const double *phi,
+                   const double *mu, int binary, long t);
 int gp_vtk_read(const char *path, const gp_input *in, double *phi, double *mu,
                 int binary);
 

@@ -251,6 +251,14 @@ int  gp_adjoint_run(gp_adjoint *adj, const double *phi0, const double *mu0, cons
    last run, and the number of forward steps taken */
 int  gp_adjoint_info(gp_adjoint *adj, double *fwd_ms, double *rev_ms, long *forward_steps);
 
+/* temp_bottom of a non-isothermal run (isothermal = 0) after t steps from the
+   state given to gp_set_state: tempgrad[0] - BASE_POS*GRADIENT with
+   BASE_POS = tempgrad[3]/dy + (tempgrad[4]/dy)*(t*dt) and
+   GRADIENT = tempgrad[1]*dy/tempgrad[2]; row y of the padded grid is at
+   temp_bottom + y*GRADIENT.  Replaces the driver's BASE_POS / temp_bottom
+   lines (microsim_gp.c:77-81, 108-113) for runs without a moving frame. */
+double gp_temp_bottom(const gp_params *p, long t);
+
 const char *gp_strerror(int code);
 
 #ifdef __cplusplus

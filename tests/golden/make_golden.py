@@ -241,6 +241,14 @@ DRIVER_CASES = [
          fill=["FILLCYLINDER = {0, 10, 12, 0, 0, 6};", "FILLCUBE = {1, 18, 20, 0, 28, 30, 0};"], gabc=1.0),
     dict(name="driver_p2c1_smooth", P=2, C=1, mesh=(40, 44), fmt="BINARY", steps=10, saveT=5, nsmooth=8,
          fill=["FILLCYLINDER = {0, 18, 22, 0, 0, 9};"]),
+    # non-isothermal, with composition and temperature blocks, then a restart
+    # from the file of step 20 (which moves the gradient by floor(v*20*dt) = 1)
+    dict(name="driver_p2c1_tempgrady_ascii", P=2, C=1, mesh=(40, 48), fmt="ASCII", steps=40, saveT=20,
+         tempgrady="{0.96, 0.06, 80.0, 5, 1.6}", composition=1, restart=(20, 20),
+         fill=["FILLCYLINDER = {0, 20, 16, 0, 0, 7};"]),
+    dict(name="driver_p3c2_tempgrady_binary", P=3, C=2, mesh=(32, 40), fmt="BINARY", steps=30, saveT=15,
+         solver="TERNARY", tempgrady="{0.94, 0.1, 60.0, 2, 2.0}", composition=1,
+         fill=["FILLCYLINDER = {0, 10, 12, 0, 0, 5};", "FILLCYLINDER = {1, 22, 26, 0, 0, 6};"]),
 ]
 
 
@@ -251,7 +259,19 @@ def run_driver_case(c, ref):
                  "WRITEFORMAT": c["fmt"], "NSMOOTH": str(c.get("nsmooth", 1))})
     if c.get("gabc") is not None:
         keys["Gamma_abc"] = "{" + ",".join([str(c["gabc"])] * (P * (P - 1) * (P - 2) // 6)) + "}"
-    infile = synth.infile_text(keys, synth.thermo_lines(P, C) + synth.bc_lines())
+    if c.get("composition"):
+        keys["Writecomposition"] = "1"
+    extra = synth.thermo_lines(P, C, solver=c.get("solver", "BINARY")) + synth.bc_lines()
+    if c.get("tempgrady"):
+        keys["ISOTHERMAL"] = "0"
+        keys["Tempgrady"] = c["tempgrady"]
+        extra += ["BOUNDARY = {T, 1, 1, 1, 1, 0, 0};", "BOUNDARY_VALUE = {T, 0, 0, 0, 0, 0, 0};"]
+    infile = synth.infile_text(keys, extra)
+    infile2 = None
+    if c.get("restart"):
+        t0, n = c["restart"]
+        keys.update({"STARTTIME": str(t0), "RESTART": "1", "NTIMESTEPS": str(n)})
+        infile2 = synth.infile_text(keys, extra)
     filling = "\n".join(c["fill"]) + "\n"
     d = tempfile.mkdtemp(prefix="gpdrv_")
     try:
@@ -261,6 +281,14 @@ def run_driver_case(c, ref):
         if r.returncode != 0:
             raise RuntimeError(f"{c['name']}: reference exited {r.returncode}")
         res = {"infile": np.array(infile), "filling": np.array(filling)}
+        if infile2 is not None:
+            # the second run reads DATA/out_<STARTTIME>.vtk and rewrites the later files
+            res["stdout_first"] = np.frombuffer(r.stdout, dtype=np.uint8)
+            res["infile_restart"] = np.array(infile2)
+            open(os.path.join(d, "Input.in"), "w").write(infile2)
+            r = subprocess.run([ref, "Input.in", "Filling.in", "out"], cwd=d, capture_output=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError(f"{c['name']}: reference restart exited {r.returncode}")
         for echo in ("Input.out", "Input.bd"):
             if os.path.exists(os.path.join(d, echo)):
                 res["echo_" + echo.replace(".", "_")] = np.frombuffer(open(os.path.join(d, echo), "rb").read(), dtype=np.uint8)

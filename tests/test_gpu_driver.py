@@ -48,6 +48,13 @@ def test_driver_reproduces_reference_files(case):
         print(f"driver case {case}", flush=True)
         r = subprocess.run([DRIVER, "Input.in", "Filling.in", "out"], cwd=d, capture_output=True, timeout=120)
         assert r.returncode == 0, r.stderr.decode()[-2000:]
+        if "infile_restart" in z:
+            # restart from DATA/out_<STARTTIME>.vtk of the first run
+            _check_report(r.stdout.decode(), bytes(z["stdout_first"]).decode(), z)
+            open(os.path.join(d, "Input.in"), "w").write(str(z["infile_restart"]))
+            print(f"driver case {case}: restart", flush=True)
+            r = subprocess.run([DRIVER, "Input.in", "Filling.in", "out"], cwd=d, capture_output=True, timeout=120)
+            assert r.returncode == 0, r.stderr.decode()[-2000:]
         files = sorted(os.listdir(os.path.join(d, "DATA")))
         assert files == list(z["files"])
         for f in files:
@@ -57,25 +64,30 @@ def test_driver_reproduces_reference_files(case):
         for ext in ("out", "bd"):
             got = open(os.path.join(d, f"Input.{ext}"), "rb").read()
             assert got == bytes(z[f"echo_Input_{ext}"]), f"{case}: Input.{ext} differs"
-        ours, ref = _report(r.stdout.decode()), _report(bytes(z["stdout"]).decode())
-        assert len(ours) == len(ref) and len(ref) > 0
-        # The reference folds extrema into running values at every step and
-        # over the interior plus the ghost rings x in {1,2,MESH_X+3},
-        # y in {1,2,MESH_Y+3,MESH_Y+4} before the boundary copy
-        # (calculate_divergence_concentration.h:128-151); the driver folds the
-        # interior extrema of the report steps only, a subset.  Likewise the
-        # squared change is summed over interior cells only.  So the driver's
-        # Max <= reference Max, Min >= reference Min, change <= reference change,
-        # with equality for the phase fractions, which stay in [0, 1].
-        nphase = int(re.search(r"NUMPHASES\s*=\s*(\d+)", str(z["infile"])).group(1))
-        for (t0, rows0), (t1, rows1) in zip(ours, ref):
-            assert t0 == t1
-            assert [x[0] for x in rows0] == [x[0] for x in rows1]
-            for i, (a, b) in enumerate(zip(rows0, rows1)):
-                if i < nphase:
-                    assert a[1:3] == b[1:3]
-                else:
-                    assert float(a[1]) <= float(b[1]) and float(a[2]) >= float(b[2])
-                    assert float(a[2]) - float(b[2]) <= 0.05 * abs(float(b[2]))
-                assert a[3] <= b[3] * (1 + 1e-9) + 1e-300
-                assert a[3] >= 0.5 * b[3]
+        _check_report(r.stdout.decode(), bytes(z["stdout"]).decode(), z)
+
+
+def _check_report(ours_text, ref_text, z):
+    """The driver's progress report against the reference's."""
+    ours, ref = _report(ours_text), _report(ref_text)
+    assert len(ours) == len(ref) and len(ref) > 0
+    # The reference folds extrema into running values at every step and
+    # over the interior plus the ghost rings x in {1,2,MESH_X+3},
+    # y in {1,2,MESH_Y+3,MESH_Y+4} before the boundary copy
+    # (calculate_divergence_concentration.h:128-151); the driver folds the
+    # interior extrema of the report steps only, a subset.  Likewise the
+    # squared change is summed over interior cells only.  So the driver's
+    # Max <= reference Max, Min >= reference Min, change <= reference change,
+    # with equality for the phase fractions, which stay in [0, 1].
+    nphase = int(re.search(r"NUMPHASES\s*=\s*(\d+)", str(z["infile"])).group(1))
+    for (t0, rows0), (t1, rows1) in zip(ours, ref):
+        assert t0 == t1
+        assert [x[0] for x in rows0] == [x[0] for x in rows1]
+        for i, (a, b) in enumerate(zip(rows0, rows1)):
+            if i < nphase:
+                assert a[1:3] == b[1:3]
+            else:
+                assert float(a[1]) <= float(b[1]) and float(a[2]) >= float(b[2])
+                assert float(a[2]) - float(b[2]) <= 0.05 * abs(float(b[2]))
+            assert a[3] <= b[3] * (1 + 1e-9) + 1e-300
+            assert a[3] >= 0.5 * b[3]
