@@ -200,13 +200,18 @@ __device__ __forceinline__ R tau_phi(const DevParams<R>& p, const R* ph) {
 // temp_bottom + Y*GRADIENT (functions/Temperature_gradient.h:17-19), where Y is
 // the row itself or, when proxy is set and the row is a ghost row with a
 // temperature boundary copy, the row it copies (boundary_serial.h:176-203).
+// In the step after a moving-frame shift by tsh rows the temperature was moved
+// with the fields (functions/shift.h:19): row Y then holds the value of row
+// Y + tsh of the profile before the shift, except the top tsh rows, which keep
+// their own.
 template <class R>
-__device__ __forceinline__ R temp_row(const DevParams<R>& p, R tb, int y, bool proxy) {
+__device__ __forceinline__ R temp_row(const DevParams<R>& p, R tb, int y, bool proxy, int tsh = 0) {
   int Y = y;
   if (proxy) {
     if (y <= 2 && p.tsrc[y] >= 0) Y = p.tsrc[y];
     else if (y >= p.ry - 3 && p.tsrc[3 + y - (p.ry - 3)] >= 0) Y = p.tsrc[3 + y - (p.ry - 3)];
   }
+  if (Y < p.ry - tsh) Y += tsh;
   return tb + (R)Y * p.tgG;
 }
 

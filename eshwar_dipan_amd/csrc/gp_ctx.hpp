@@ -26,6 +26,9 @@ struct gp_ctx {
   int cur;              // index of the current state buffers
   int stepped;          // the other buffers hold the state one step earlier
   long tstep;           // steps since the state was loaded (temperature of non-isothermal runs)
+  long shift_off;       // rows the moving frame has moved since the state was loaded
+  long shift_pend;      // rows of a shift not yet followed by a step (temperature of that step)
+  int *drow;            // interface position per interior row (allocated on first use)
   int path;             // 0 fused, 1 two-pass
   hipStream_t stream;
   hipEvent_t ev0, ev1;

@@ -745,11 +745,10 @@ static double to_big_endian(double v) {
   return v;
 }
 
-/* temp_bottom after t steps of a non-isothermal run without moving frame
-   (microsim_gp.c:78-80 for t = 0, 109-111 after step t; shift_OFFSET = 0) */
-double gp_temp_bottom(const gp_params *p, long t) {
+/* temp_bottom after t steps of a non-isothermal run whose frame has moved by
+   shift_OFFSET rows (microsim_gp.c:78-80 for t = 0, 109-111 after step t) */
+double gp_temp_bottom(const gp_params *p, long shift_OFFSET, long t) {
   const double G = (p->tempgrad[1]) * p->dy / (p->tempgrad[2]);
-  const long shift_OFFSET = 0;
   double base = (p->tempgrad[3] / p->dy) - shift_OFFSET;
   if (t > 0) base = (p->tempgrad[3] / p->dy) - shift_OFFSET + ((p->tempgrad[4] / p->dy) * (t * p->dt));
   return p->tempgrad[0] - base * G;
@@ -761,19 +760,27 @@ static void put_value(FILE *f, double v, int binary, const char *fmt) {
 }
 
 int gp_vtk_write(const char *path, const gp_input *in, const double *phi, const double *mu, int binary) {
-  return gp_vtk_write_t(path, in, phi, mu, binary, 0);
+  return gp_vtk_write_t(path, in, phi, mu, binary, 0, 0, 0);
 }
 
 /* write_cells_vtk_2D / write_cells_vtk_2D_binary (solverloop/file_writer.h:
    60-253): phases, Mu_<k>, Composition_<k> when WRITECOMPOSITION, and for a
    non-isothermal run the Temperature block.  The temperature of interior row
    y (padded index) after t steps is temp_bottom + y*GRADIENT
-   (functions/Temperature_gradient.h:15-19), which the composition also uses. */
-int gp_vtk_write_t(const char *path, const gp_input *in, const double *phi, const double *mu, int binary, long t) {
+   (functions/Temperature_gradient.h:15-19), which the composition also uses;
+   right after a shift by d rows, row y holds the value of row y + d of the
+   profile before the shift, except the top d rows (functions/shift.h:19). */
+static double row_temp(double tb, double G, long y, long ry, long d) {
+  return tb + (double)(y < ry - d ? y + d : y) * G;
+}
+
+int gp_vtk_write_t(const char *path, const gp_input *in, const double *phi, const double *mu, int binary, long t,
+                   long shift_offset, long shifted) {
   const gp_params *p = &in->p;
   const int ni = !p->isothermal;
-  const double tb = ni ? gp_temp_bottom(p, t) : 0.0;
+  const double tb = ni ? gp_temp_bottom(p, shift_offset - shifted, t) : 0.0;
   const double G = ni ? (p->tempgrad[1]) * p->dy / (p->tempgrad[2]) : 0.0;
+  const long ry = p->mesh_y + 6;
   FILE *f = fopen(path, binary ? "wb" : "w");
   if (!f) return GP_E_IO;
   long MX = p->mesh_x, MY = p->mesh_y;
@@ -814,7 +821,7 @@ int gp_vtk_write_t(const char *path, const gp_input *in, const double *phi, cons
         double ph[P_MAX], m[C_MAX], comp = 0.0;
         for (int a = 0; a < p->nphases; a++) ph[a] = phi[a * n + i];
         for (int l = 0; l < p->ncomp; l++) m[l] = mu[l * n + i];
-        const double T = ni ? tb + (double)(i % MY + 3) * G : p->T;
+        const double T = ni ? row_temp(tb, G, (long)(i % MY) + 3, ry, shifted) : p->T;
         for (int b = 0; b < p->nphases; b++) comp += c_of_mu(p, m, T, b, k) * h_of_phi(ph, p->nphases, b);
         put_value(f, comp, binary, "%le \n");
       }
@@ -824,7 +831,7 @@ int gp_vtk_write_t(const char *path, const gp_input *in, const double *phi, cons
   if (ni) {
     fprintf(f, "SCALARS Temperature double 1\n");
     fprintf(f, "LOOKUP_TABLE default\n");
-    for (size_t i = 0; i < n; i++) put_value(f, tb + (double)(i % MY + 3) * G, binary, "%le \n");
+    for (size_t i = 0; i < n; i++) put_value(f, row_temp(tb, G, (long)(i % MY) + 3, ry, shifted), binary, "%le \n");
   }
   int bad = ferror(f);
   fclose(f);

@@ -214,20 +214,21 @@ __device__ __forceinline__ int conc_cell(const DevParams<R>& p, const R* __restr
 // delta-phi on [2, rows-3]^2, one cell per thread
 __host__ __device__ inline dim3 phase_grid(int rx, int ry) { return dim3((ry - 4 + BX - 1) / BX, (rx - 4 + BY - 1) / BY); }
 
-// With NI (non-isothermal), tb is temp_bottom of this step and tproxy tells
+// With NI (non-isothermal), tb is temp_bottom of this step; bit 0 of tmode tells
 // whether the temperature ghost rows hold their boundary copies (first step
-// after the copies) or the linear profile.
+// after the copies) or the linear profile, and tmode >> 1 is the row shift of
+// the step after a moving-frame shift (temp_row).
 template <int P, int C, class R, bool NI = false>
 __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                    const R* __restrict__ mu, R* __restrict__ dphi, R tb = R(0),
-                                                   int tproxy = 0) {
+                                                   int tmode = 0) {
   const DevParams<R>& p = *prm;
   int bx, by;
   blk_xy(bx, by);
   const int y = 2 + bx * BX + threadIdx.x;
   const int x = 2 + by * BY + threadIdx.y;
   if (x > p.rx - 3 || y > p.ry - 3) return;
-  phase_cell<P, C, R, NI>(p, phi, mu, dphi, x, y, NI ? temp_row(p, tb, y, tproxy != 0) : R(0));
+  phase_cell<P, C, R, NI>(p, phi, mu, dphi, x, y, NI ? temp_row(p, tb, y, (tmode & 1) != 0, tmode >> 1) : R(0));
 }
 
 // mu and phi update of the interior with shared faces.  The antitrapping flux of a face is the same
@@ -253,7 +254,7 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
                                                               const R* __restrict__ phi, const R* __restrict__ mu,
                                                               const R* __restrict__ dphi, const R* __restrict__ mulo,
                                                               R* __restrict__ phi_o, R* __restrict__ mu_o,
-                                                              R* __restrict__ mulo_o, R tb = R(0)) {
+                                                              R* __restrict__ mulo_o, R tb = R(0), int tsh = 0) {
   const DevParams<R>& p = *prm;
   const int lane = threadIdx.x;
   int bx, by;
@@ -261,7 +262,7 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
   const int y = 2 + bx * MW + lane;  // lane 0 sits one column before the wave's cells
   const int yc = min(y, p.mesh_y + 2);
   // temperatures of the cell row and of the row above (the sweep runs after the boundary copies)
-  const R Tc = NI ? temp_row(p, tb, yc, true) : R(0), Tr = NI ? temp_row(p, tb, yc + 1, true) : R(0);
+  const R Tc = NI ? temp_row(p, tb, yc, true, tsh) : R(0), Tr = NI ? temp_row(p, tb, yc + 1, true, tsh) : R(0);
   const bool cell = lane > 0 && y <= p.mesh_y + 2;
   const int x0 = 3 + (by * BY + threadIdx.y) * KM;
   if (x0 > p.mesh_x + 2) return;  // uniform over the wave
@@ -344,7 +345,7 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
                                                             const R* __restrict__ phi, const R* __restrict__ mu,
                                                             const R* __restrict__ dphi, const R* __restrict__ mulo,
                                                             R* __restrict__ phi_o, R* __restrict__ mu_o,
-                                                            R* __restrict__ mulo_o, R tb = R(0)) {
+                                                            R* __restrict__ mulo_o, R tb = R(0), int tsh = 0) {
   __shared__ R jx[C][BY][BX];
   const DevParams<R>& p = *prm;
   const int lane = threadIdx.x, w = threadIdx.y;
@@ -353,7 +354,7 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
   const int y = 2 + bx * MW + lane;  // lane 0 sits one column before the wave's cells
   const int yc = min(y, p.mesh_y + 2);
   // temperatures of the cell row and of the row above (the sweep runs after the boundary copies)
-  const R Tc = NI ? temp_row(p, tb, yc, true) : R(0), Tr = NI ? temp_row(p, tb, yc + 1, true) : R(0);
+  const R Tc = NI ? temp_row(p, tb, yc, true, tsh) : R(0), Tr = NI ? temp_row(p, tb, yc + 1, true, tsh) : R(0);
   const int x = 3 + by * BY + w;
   const int xc = min(x, p.mesh_x + 2);       // waves past the last row compute a copy and write nothing
   const bool cell = x <= p.mesh_x + 2 && lane > 0 && y <= p.mesh_y + 2;
@@ -412,23 +413,25 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
 // the barrier costs more than the face it saves (1% slower), so one row per
 // wave.  The sharing kernels beat one cell per thread also at two phases
 // (512^2: 1.3% in binary64, 5% in binary32; equal at 256^2).
-// With NI (non-isothermal), tb is temp_bottom of the step and tproxy whether the
-// phase sweep sees the temperature boundary copies (first step after them).
+// With NI (non-isothermal), tb is temp_bottom of the step, tproxy whether the
+// phase sweep sees the temperature boundary copies (first step after them) and
+// tsh the row shift of the step after a moving-frame shift.
 template <int P, int C, class R, bool NI = false>
 void launch_two_pass(const DevParams<R>* prm, int rx, int ry, int mesh_x, int mesh_y, hipStream_t stream,
                      const R* phi, const R* mu, R* dphi, const R* mulo, R* phi_o, R* mu_o, R* mulo_o,
-                     R tb = R(0), int tproxy = 0) {
+                     R tb = R(0), int tproxy = 0, int tsh = 0) {
   const dim3 blk(BX, BY);
-  hipLaunchKernelGGL((k_phase<P, C, R, NI>), phase_grid(rx, ry), blk, 0, stream, prm, phi, mu, dphi, tb, tproxy);
+  hipLaunchKernelGGL((k_phase<P, C, R, NI>), phase_grid(rx, ry), blk, 0, stream, prm, phi, mu, dphi, tb,
+                     (tproxy ? 1 : 0) | (tsh << 1));
   constexpr bool lds = Ops<R>::Exact;
   constexpr int rows = BY * (lds ? 1 : KM);
   const dim3 g((mesh_y + MW - 1) / MW, (mesh_x + rows - 1) / rows);
   if constexpr (lds)
     hipLaunchKernelGGL((k_conc_lds<P, C, R, NI>), g, blk, 0, stream, prm, phi, mu, (const R*)dphi, mulo, phi_o, mu_o,
-                       mulo_o, tb);
+                       mulo_o, tb, tsh);
   else
     hipLaunchKernelGGL((k_conc_march<P, C, R, NI>), g, blk, 0, stream, prm, phi, mu, (const R*)dphi, mulo, phi_o,
-                       mu_o, mulo_o, tb);
+                       mu_o, mulo_o, tb, tsh);
 }
 
 // Fill the ghost layers of a freshly loaded state: every interior cell copies
@@ -447,6 +450,51 @@ __global__ void k_ghosts(const DevParams<R>* __restrict__ prm, R* __restrict__ p
     R* b = mu + (size_t)k * p.plane;
     write_with_ghosts(p, 1, b, x, y, b[(size_t)x * p.pitch + y]);
   }
+}
+
+// Moving frame (apply_shiftY, functions/shift.h:4-38): every padded column,
+// ghost layers included, moves d rows towards y = 0 and the top d rows take
+// the fill value of their plane.  Out of place, nf planes from src to dst.
+struct ShiftFill {
+  double v[GP_MAX_PHASES > GP_MAX_COMP ? GP_MAX_PHASES : GP_MAX_COMP];
+};
+template <class R>
+__global__ void k_shift_y(const DevParams<R>* __restrict__ prm, const R* __restrict__ src, R* __restrict__ dst, int nf,
+                          int d, ShiftFill f) {
+  const DevParams<R>& p = *prm;
+  const int y = blockIdx.x * BX + threadIdx.x;
+  const int x = blockIdx.y * BY + threadIdx.y;
+  if (x >= p.rx || y >= p.ry) return;
+  for (int k = 0; k < nf; k++) {
+    const size_t o = (size_t)k * p.plane + (size_t)x * p.pitch + y;
+    dst[o] = y + d < p.ry ? src[o + d] : (R)f.v[k];
+  }
+}
+
+// Interface position of the moving frame (check_SHIFT, functions/shift.h:44-66)
+// over the interior rows: per row x, the largest padded y in [1, ry-1] at which
+// the last phase goes from below 1/2 at y-1 to above 1/2 at y, 0 if none.  One
+// block of IFB threads per row; the row result goes to out[x - 3].
+constexpr int IFB = 256;
+template <class R>
+__global__ void __launch_bounds__(IFB) k_iface_rows(const DevParams<R>* __restrict__ prm, const R* __restrict__ phil,
+                                                    int* __restrict__ out) {
+  __shared__ int red[IFB];
+  const DevParams<R>& p = *prm;
+  const int x = 3 + blockIdx.x;
+  const R* row = phil + (size_t)x * p.pitch;
+  int best = 0;
+  for (int y = 1 + threadIdx.x; y < p.ry; y += IFB) {
+    const R a = row[y - 1], b = row[y];
+    if ((a - (R(1) - a) < R(0)) && (b - (R(1) - b) > R(0))) best = max(best, y);
+  }
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int s = IFB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 // interior <-> padded copies (host layout [f][MX][MY] dense, R or double)

@@ -205,10 +205,10 @@ int run_sparse(gp_ctx *c, long nsteps) {
   return GP_OK;
 }
 
-// temp_bottom after t steps of a non-isothermal run, in binary64 as the
-// reference forms it (microsim_gp.c:78-80,109-111; no moving frame, so
-// shift_OFFSET = 0)
-inline double temp_bottom(const gp_params &h, long t) { return gp_temp_bottom(&h, t); }
+// temp_bottom after t steps of a non-isothermal run whose frame has moved by
+// shift_off rows, in binary64 as the reference forms it (microsim_gp.c:78-80,
+// 109-111)
+inline double temp_bottom(const gp_params &h, long shift_off, long t) { return gp_temp_bottom(&h, shift_off, t); }
 
 template <int P, int C, class R>
 int run_steps(gp_ctx *c, long nsteps) {
@@ -234,15 +234,20 @@ int run_steps(gp_ctx *c, long nsteps) {
                                (R *)c->mulo[c->cur ^ 1]);
     } else {
       // step k = tstep + t + 1 sees the temperature set after step k-1; its phase
-      // sweep sees the temperature boundary copies only in the first step
+      // sweep sees the temperature boundary copies only in the first step.  A
+      // step right after a shift of the frame sees that profile moved with the
+      // fields, in the frame before the shift.
       const long k = c->tstep + t;
+      const long tsh = t == 0 ? c->shift_pend : 0;
       launch_two_pass<P, C, R, true>(prm, c->rx, c->ry, (int)c->hp.mesh_x, (int)c->hp.mesh_y, c->stream,
                                      (const R *)c->phi[c->cur], (const R *)c->mu[c->cur], (R *)c->dphi,
                                      (const R *)c->mulo[c->cur], (R *)c->phi[c->cur ^ 1], (R *)c->mu[c->cur ^ 1],
-                                     (R *)c->mulo[c->cur ^ 1], (R)temp_bottom(c->hp, k), k == 0 ? 1 : 0);
+                                     (R *)c->mulo[c->cur ^ 1], (R)temp_bottom(c->hp, c->shift_off - tsh, k),
+                                     k == 0 ? 1 : 0, (int)tsh);
     }
     c->cur ^= 1;
   }
+  if (nsteps > 0) c->shift_pend = 0;
   c->flags_valid = 0;
   GP_HIP_CHECK(hipGetLastError());
   return GP_OK;
@@ -272,7 +277,7 @@ int run_stats(gp_ctx *c, gp_stats *st) {
   const R *mp = c->stepped ? (const R *)c->mu[c->cur ^ 1] : nullptr;
   hipLaunchKernelGGL((k_stats<P, C, R>), dim3(c->nstat_blocks), dim3(256), 0, c->stream, prm,
                      (const R *)c->phi[c->cur], (const R *)c->mu[c->cur], pp, mp, c->dpart,
-                     c->hp.isothermal ? 0.0 : temp_bottom(c->hp, c->tstep));
+                     c->hp.isothermal ? 0.0 : temp_bottom(c->hp, c->shift_off, c->tstep));
   GP_HIP_CHECK(hipGetLastError());
   std::vector<double> h((size_t)c->nstat_blocks * NV);
   GP_HIP_CHECK(hipMemcpyAsync(h.data(), c->dpart, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));

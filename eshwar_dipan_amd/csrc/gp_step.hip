@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 #include <new>
 
 #include "gp_ctx.hpp"
@@ -151,6 +152,8 @@ static int load_state(gp_ctx *c, const S *dphi_src, const S *dmu_src) {
   c->flags_valid = 0;
   c->stepped = 0;
   c->tstep = 0;
+  c->shift_off = 0;
+  c->shift_pend = 0;
   return GP_OK;
 }
 
@@ -191,6 +194,7 @@ static void free_ctx(gp_ctx *c) {
   if (c->dcode) (void)hipFree(c->dcode);
   if (c->dprm) (void)hipFree(c->dprm);
   if (c->dpart) (void)hipFree(c->dpart);
+  if (c->drow) (void)hipFree(c->drow);
   for (int i = 0; i < 2; i++)
     if (c->dflag[i]) (void)hipFree(c->dflag[i]);
   if (c->ddeps) (void)hipFree(c->ddeps);
@@ -498,6 +502,7 @@ int gp_step_impl(gp_ctx *c, long nsteps, bool timed) {
   if (rc) return rc;
   if (nsteps > 0) c->stepped = 1;
   c->tstep += nsteps;
+  if (nsteps > 0) c->shift_pend = 0;
   for (int i = 0; i < c->nmember; i++) {
     c->member[i]->cur = c->cur;
     c->member[i]->stepped = c->stepped;
@@ -628,6 +633,68 @@ int gp_stats_get(gp_ctx *c, gp_stats *st) {
 #undef GP_DISPATCH_STATS
   if (rc) return rc;
   return st->nonfinite > 0 ? GP_E_NONFINITE : GP_OK;
+}
+
+// -------------------------------------------------------------- moving frame
+
+}  // extern "C"
+
+template <class R>
+static int frame_pos(gp_ctx *c, long *pos) {
+  const gpk::DevParams<R> *prm = (const gpk::DevParams<R> *)c->dprm;
+  const long MX = c->hp.mesh_x;
+  if (!c->drow && hipMalloc((void **)&c->drow, sizeof(int) * MX) != hipSuccess) {
+    c->drow = nullptr;
+    return GP_E_NOMEM;
+  }
+  hipLaunchKernelGGL((gpk::k_iface_rows<R>), dim3((unsigned)MX), dim3(gpk::IFB), 0, c->stream, prm,
+                     (const R *)c->phi[c->cur] + (size_t)(c->P - 1) * c->plane, c->drow);
+  GP_HIP_CHECK(hipGetLastError());
+  std::vector<int> h((size_t)MX);
+  GP_HIP_CHECK(hipMemcpyAsync(h.data(), c->drow, sizeof(int) * MX, hipMemcpyDeviceToHost, c->stream));
+  GP_HIP_CHECK(hipStreamSynchronize(c->stream));
+  int m = 0;
+  for (int v : h) m = std::max(m, v);
+  *pos = m;
+  return GP_OK;
+}
+
+template <class R>
+static int frame_shift(gp_ctx *c, int d, const double *mu_fill) {
+  const gpk::DevParams<R> *prm = (const gpk::DevParams<R> *)c->dprm;
+  const dim3 blk(gpk::BX, gpk::BY), g((c->ry + gpk::BX - 1) / gpk::BX, (c->rx + gpk::BY - 1) / gpk::BY);
+  gpk::ShiftFill fp = {}, fm = {}, fz = {};
+  fp.v[c->P - 1] = 1.0;
+  for (int k = 0; k < c->C; k++) fm.v[k] = mu_fill[k];
+  const int o = c->cur ^ 1;
+  hipLaunchKernelGGL((gpk::k_shift_y<R>), g, blk, 0, c->stream, prm, (const R *)c->phi[c->cur], (R *)c->phi[o], c->P, d, fp);
+  hipLaunchKernelGGL((gpk::k_shift_y<R>), g, blk, 0, c->stream, prm, (const R *)c->mu[c->cur], (R *)c->mu[o], c->C, d, fm);
+  if (c->mulo[c->cur])
+    hipLaunchKernelGGL((gpk::k_shift_y<R>), g, blk, 0, c->stream, prm, (const R *)c->mulo[c->cur], (R *)c->mulo[o], c->C, d, fz);
+  GP_HIP_CHECK(hipGetLastError());
+  c->cur = o;
+  return GP_OK;
+}
+
+extern "C" {
+
+int gp_check_shift(gp_ctx *c, long *pos) {
+  if (!c || !pos || c->nmember > 0 || !c->phi[0]) return GP_E_ARG;
+  return c->dtype == GP_F64 ? frame_pos<double>(c, pos) : frame_pos<float>(c, pos);
+}
+
+int gp_apply_shift_y(gp_ctx *c, long d, const double *mu_fill) {
+  if (!c || !mu_fill || c->nmember > 0 || !c->phi[0]) return GP_E_ARG;
+  if (d < 1 || d > c->hp.mesh_y) return GP_E_ARG;
+  for (int k = 0; k < c->C; k++)
+    if (!isfinite(mu_fill[k])) return GP_E_ARG;
+  int rc = c->dtype == GP_F64 ? frame_shift<double>(c, (int)d, mu_fill) : frame_shift<float>(c, (int)d, mu_fill);
+  if (rc) return rc;
+  c->shift_off += d;
+  c->shift_pend += d;
+  c->stepped = 0;
+  c->flags_valid = 0;
+  return GP_OK;
 }
 
 // ------------------------------------------------------------------ ensembles

@@ -13,10 +13,11 @@
  *
  * Environment: GP_DTYPE=f32 selects the binary32 step (default binary64).
  * Non-isothermal runs (ISOTHERMAL = 0, Tempgrady) step with the row
- * temperature of the reference; their files carry the Temperature block.
+ * temperature of the reference; their files carry the Temperature block.  With
+ * Shift = 1 the frame moves every 100 steps (microsim_gp.c:115-126).
  * Differences from the reference are reported instead of silently diverging:
- * runs that need a moving frame (Shift), phase-field noise, or smoothing
- * sweeps of a non-isothermal run exit with a message and status 2.
+ * runs that need phase-field noise, or smoothing sweeps of a non-isothermal
+ * run, exit with a message and status 2.
  */
 #include <math.h>
 #include <stdio.h>
@@ -54,8 +55,8 @@ int main(int argc, char *argv[]) {
     fprintf(stderr, "microsim_gp: %s\n", err);
     return 2;
   }
-  if (in.shift || in.noise_phasefield) {
-    fprintf(stderr, "microsim_gp: Shift and Noise_phasefield runs are not stepped by this build\n");
+  if (in.noise_phasefield) {
+    fprintf(stderr, "microsim_gp: Noise_phasefield runs are not stepped by this build\n");
     return 2;
   }
   if (!in.p.isothermal && in.nsmooth > 1) {
@@ -74,6 +75,7 @@ int main(int argc, char *argv[]) {
   if (!phi || !mu) return fail("allocating host fields", GP_E_NOMEM);
 
   char name[1200];
+  long shift_position = 0, shift_OFFSET = 0;
   if (in.starttime == 0 && in.restart == 0) {
     rc = gp_fill_domain(&in, argv[2], phi, mu, 1, err, sizeof err);
     if (rc) {
@@ -84,10 +86,26 @@ int main(int argc, char *argv[]) {
     snprintf(name, sizeof name, "DATA/%s_%ld.vtk", argv[3], in.starttime);
     rc = gp_vtk_read(name, &in, phi, mu, !in.ascii);
     if (rc) return fail(name, rc);
-    /* a restart moves the gradient by the distance travelled up to STARTTIME
-       (microsim_gp.c:67-74, no moving frame) */
-    if (!in.p.isothermal)
-      in.p.tempgrad[3] = (in.p.tempgrad[3]) + floor((in.p.tempgrad[4]) * (in.starttime * in.p.dt));
+    if (in.shift) {
+      /* frame position of the restart file: the reference reads
+         STARTTIME/SAVET + 1 lines of DATA/shift.dat and keeps the last one
+         read (microsim_gp.c:57-66) */
+      FILE *fp = fopen("DATA/shift.dat", "r");
+      if (!fp) return fail("DATA/shift.dat", GP_E_IO);
+      long time_file = 0, position = 0;
+      for (long file_iter = 0; file_iter <= in.starttime / in.saveT; file_iter++)
+        if (fscanf(fp, "%ld %ld\n", &time_file, &position) != 2) break;
+      fclose(fp);
+      shift_position = position;
+    }
+    /* a restart moves the gradient by the distance travelled up to STARTTIME,
+       less the frame position (microsim_gp.c:67-74) */
+    if (!in.p.isothermal) {
+      if (in.shift)
+        in.p.tempgrad[3] = (in.p.tempgrad[3]) + floor((in.p.tempgrad[4]) * (in.starttime * in.p.dt)) - shift_position * in.p.dy;
+      else
+        in.p.tempgrad[3] = (in.p.tempgrad[3]) + floor((in.p.tempgrad[4]) * (in.starttime * in.p.dt));
+    }
   }
 
   gp_ctx *ctx = NULL;
@@ -120,36 +138,57 @@ int main(int argc, char *argv[]) {
   for (int a = 0; a < P; a++) run_phi_max[a] = 1.0, run_phi_min[a] = 0.0;
   for (int k = 0; k < C; k++) run_mu_max[k] = 1.0, run_mu_min[k] = 1.0;
 
+  /* rows entering the moving frame are liquid at cfill of the liquid and
+     Tfill (functions/shift.h:24-36) */
+  double mu_fill[GP_MAX_COMP];
+  for (int k = 0; k < C; k++) mu_fill[k] = gp_mu_of_c(p, p->cfill[P - 1][P - 1], in.Tfill, P - 1, k);
+
   long t = 0;
   while (t < in.ntimesteps) {
-    /* advance to the next save or report point in one call */
+    /* advance to the next save, report or shift point in one call */
     long next = in.ntimesteps;
     long ns = (t / in.saveT + 1) * in.saveT, nr = (t / in.track_progress + 1) * in.track_progress;
     if (ns < next) next = ns;
     if (nr < next) next = nr;
+    if (in.shift && (t / 100 + 1) * 100 < next) next = (t / 100 + 1) * 100;
     rc = gp_step(ctx, next - t);
     if (rc) return fail("stepping", rc);
     t = next;
-    if (t % in.saveT == 0) {
-      rc = gp_get_state(ctx, phi, mu);
-      if (rc) return fail("reading back the state", rc);
-      snprintf(name, sizeof name, "DATA/%s_%ld.vtk", argv[3], t + in.starttime);
-      rc = gp_vtk_write_t(name, &in, phi, mu, !in.ascii, t);
-      if (rc) return fail(name, rc);
-      FILE *fp = fopen("DATA/shift.dat", "a");
-      if (fp) {
-        fprintf(fp, "%ld %ld\n", t + in.starttime, (long)0);
-        fclose(fp);
-      }
-    }
+    /* the report's extrema and changes are those of the step, before a shift */
+    gp_stats st;
     if (t % in.track_progress == 0) {
-      gp_stats st;
       rc = gp_stats_get(ctx, &st);
       if (rc == GP_E_NONFINITE) {
         fprintf(stderr, "microsim_gp: non-finite values after step %ld\n", t);
         return 3;
       }
       if (rc) return fail("statistics", rc);
+    }
+    long shifted = 0;
+    if (in.shift && t % 100 == 0) {
+      long pos = 0;
+      rc = gp_check_shift(ctx, &pos);
+      if (rc) return fail("locating the interface", rc);
+      if (pos > in.shiftj) {
+        shifted = pos - in.shiftj;
+        rc = gp_apply_shift_y(ctx, shifted, mu_fill);
+        if (rc) return fail("shifting the frame", rc);
+        shift_OFFSET += shifted;
+      }
+    }
+    if (t % in.saveT == 0) {
+      rc = gp_get_state(ctx, phi, mu);
+      if (rc) return fail("reading back the state", rc);
+      snprintf(name, sizeof name, "DATA/%s_%ld.vtk", argv[3], t + in.starttime);
+      rc = gp_vtk_write_t(name, &in, phi, mu, !in.ascii, t, shift_OFFSET, shifted);
+      if (rc) return fail(name, rc);
+      FILE *fp = fopen("DATA/shift.dat", "a");
+      if (fp) {
+        fprintf(fp, "%ld %ld\n", t + in.starttime, shift_OFFSET + shift_position);
+        fclose(fp);
+      }
+    }
+    if (t % in.track_progress == 0) {
       for (int a = 0; a < P; a++) {
         if (st.phi_max[a] > run_phi_max[a]) run_phi_max[a] = st.phi_max[a];
         if (st.phi_min[a] < run_phi_min[a]) run_phi_min[a] = st.phi_min[a];

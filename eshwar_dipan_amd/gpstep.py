@@ -48,7 +48,8 @@ def lib():
             ("gp_check_supported", [ct.POINTER(GpParams), ct.c_char_p, ct.c_size_t]),
             ("gp_fill_domain", [ct.POINTER(GpInput), ct.c_char_p, vp, vp, i, ct.c_char_p, ct.c_size_t]),
             ("gp_vtk_write", [ct.c_char_p, ct.POINTER(GpInput), vp, vp, i]),
-            ("gp_vtk_write_t", [ct.c_char_p, ct.POINTER(GpInput), vp, vp, i, l]),
+            ("gp_vtk_write_t", [ct.c_char_p, ct.POINTER(GpInput), vp, vp, i, l, l, l]),
+            ("gp_check_shift", [vp, ct.POINTER(l)]), ("gp_apply_shift_y", [vp, l, vp]),
             ("gp_vtk_read", [ct.c_char_p, ct.POINTER(GpInput), vp, vp, i]),
             ("gp_matinv", [vp, vp, i]),
             ("gp_abi_sizes", [ct.POINTER(ct.c_size_t)] * 3),
@@ -138,19 +139,22 @@ def vtk_read(path, inp, binary=True):
     return phi, mu
 
 
-def vtk_write(path, inp, phi, mu, binary=True, step=0):
-    """Legacy VTK of the state after `step` steps (the step sets the row
-    temperature of a non-isothermal input; isothermal files do not depend on it)."""
+def vtk_write(path, inp, phi, mu, binary=True, step=0, shift_offset=0, shifted=0):
+    """Legacy VTK of the state after `step` steps, the frame moved by
+    `shift_offset` rows (`shifted` of them right after that step).  These set
+    the row temperature of a non-isothermal input; isothermal files do not
+    depend on them."""
     phi = np.ascontiguousarray(phi, dtype=np.float64)
     mu = np.ascontiguousarray(mu, dtype=np.float64)
-    _check(lib().gp_vtk_write_t(os.fsencode(path), ct.byref(inp), _ptr(phi), _ptr(mu), int(binary), int(step)), path)
+    _check(lib().gp_vtk_write_t(os.fsencode(path), ct.byref(inp), _ptr(phi), _ptr(mu), int(binary), int(step),
+                                int(shift_offset), int(shifted)), path)
 
 
-def temp_bottom(params, step):
+def temp_bottom(params, step, shift_offset=0):
     """Temperature of padded row 0 of a non-isothermal run after `step` steps."""
     f = lib().gp_temp_bottom
-    f.argtypes, f.restype = [ct.POINTER(GpParams), ct.c_long], ct.c_double
-    return f(ct.byref(params), int(step))
+    f.argtypes, f.restype = [ct.POINTER(GpParams), ct.c_long, ct.c_long], ct.c_double
+    return f(ct.byref(params), int(shift_offset), int(step))
 
 
 class Solver:
@@ -221,6 +225,17 @@ class Solver:
     def smooth(self, nsweeps):
         """nsweeps pre-loop smoothing sweeps (an Input.in's NSMOOTH - 1)."""
         _check(lib().gp_smooth(self._h, int(nsweeps)), "gp_smooth")
+
+    def check_shift(self):
+        """Interface position of the moving frame (check_SHIFT over the interior rows)."""
+        pos = ct.c_long()
+        _check(lib().gp_check_shift(self._h, ct.byref(pos)), "gp_check_shift")
+        return pos.value
+
+    def apply_shift_y(self, d, mu_fill):
+        """Move the frame d rows (apply_shiftY); new top rows are liquid at mu_fill."""
+        m = np.ascontiguousarray(mu_fill, dtype=np.float64)
+        _check(lib().gp_apply_shift_y(self._h, int(d), _ptr(m)), "gp_apply_shift_y")
 
     def synchronize(self):
         _check(lib().gp_synchronize(self._h), "gp_synchronize")
@@ -359,6 +374,17 @@ class Ensemble:
     def smooth(self, nsweeps):
         """nsweeps pre-loop smoothing sweeps (an Input.in's NSMOOTH - 1)."""
         _check(lib().gp_smooth(self._h, int(nsweeps)), "gp_smooth")
+
+    def check_shift(self):
+        """Interface position of the moving frame (check_SHIFT over the interior rows)."""
+        pos = ct.c_long()
+        _check(lib().gp_check_shift(self._h, ct.byref(pos)), "gp_check_shift")
+        return pos.value
+
+    def apply_shift_y(self, d, mu_fill):
+        """Move the frame d rows (apply_shiftY); new top rows are liquid at mu_fill."""
+        m = np.ascontiguousarray(mu_fill, dtype=np.float64)
+        _check(lib().gp_apply_shift_y(self._h, int(d), _ptr(m)), "gp_apply_shift_y")
 
     def synchronize(self):
         _check(lib().gp_ensemble_synchronize(self._h), "gp_ensemble_synchronize")

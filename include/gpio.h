@@ -77,12 +77,15 @@ int gp_fill_domain(const gp_input *in, const char *filling_path,
    The composition block (Writecomposition) is written when in->writecomposition. */
 int gp_vtk_write(const char *path, const gp_input *in, const double *phi,
                  const double *mu, int binary);
-/* gp_vtk_write for the state after t steps of this run: for a non-isothermal
-   input the composition uses the row temperature gp_temp_bottom(p, t) +
-   y*GRADIENT and a Temperature block follows (file_writer.h:137-148, 238-252).
-   gp_vtk_write is this call with t = 0. */
+/* gp_vtk_write for the state after t steps of this run, with the frame moved
+   by shift_offset rows, the last `shifted` of them right after step t: for a
+   non-isothermal input the composition uses the row temperature and a
+   Temperature block follows (file_writer.h:137-148, 238-252).  The row
+   temperature is gp_temp_bottom(p, shift_offset, t) + y*GRADIENT, or, right
+   after a shift, the profile before the shift moved with the fields
+   (functions/shift.h:19).  gp_vtk_write is this call with t = 0 and no shift. */
 int gp_vtk_write_t(const char *path, const gp_input *in, const double *phi,
-                   const double *mu, int binary, long t);
+                   const double *mu, int binary, long t, long shift_offset, long shifted);
 int gp_vtk_read(const char *path, const gp_input *in, double *phi, double *mu,
                 int binary);
 

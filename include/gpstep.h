@@ -252,12 +252,36 @@ int  gp_adjoint_run(gp_adjoint *adj, const double *phi0, const double *mu0, cons
 int  gp_adjoint_info(gp_adjoint *adj, double *fwd_ms, double *rev_ms, long *forward_steps);
 
 /* temp_bottom of a non-isothermal run (isothermal = 0) after t steps from the
-   state given to gp_set_state: tempgrad[0] - BASE_POS*GRADIENT with
-   BASE_POS = tempgrad[3]/dy + (tempgrad[4]/dy)*(t*dt) and
-   GRADIENT = tempgrad[1]*dy/tempgrad[2]; row y of the padded grid is at
-   temp_bottom + y*GRADIENT.  Replaces the driver's BASE_POS / temp_bottom
-   lines (microsim_gp.c:77-81, 108-113) for runs without a moving frame. */
-double gp_temp_bottom(const gp_params *p, long t);
+   state given to gp_set_state, with the frame moved by shift_offset rows:
+   tempgrad[0] - BASE_POS*GRADIENT with
+   BASE_POS = tempgrad[3]/dy - shift_offset + (tempgrad[4]/dy)*(t*dt) (the last
+   term only for t > 0) and GRADIENT = tempgrad[1]*dy/tempgrad[2]; row y of the
+   padded grid is at temp_bottom + y*GRADIENT.  Replaces the driver's BASE_POS /
+   temp_bottom lines (microsim_gp.c:77-81, 108-113). */
+double gp_temp_bottom(const gp_params *p, long shift_offset, long t);
+
+/*
+ * Moving frame in y (Shift = 1).
+ * gp_check_shift: check_SHIFT of the current state (functions/shift.h:44-66,
+ * folded over rows as solverloop/solverloop.h:61-66 does): the largest padded
+ * y in [1, MESH_Y+5] at which the last phase goes from phi < 1/2 at y-1 to
+ * phi > 1/2 at y, over the interior rows x in [3, MESH_X+2]; 0 if there is
+ * none.  The reference also scans the ghost rows x = 1, 2, MESH_X+3, which
+ * hold values of its sweep before the boundary copy.
+ * gp_apply_shift_y: apply_shiftY (functions/shift.h:4-38) by d rows,
+ * 1 <= d <= MESH_Y: every padded column, ghost layers included, moves d rows
+ * towards y = 0; the top d rows become the last phase (phi = 1, the others 0)
+ * at mu = mu_fill[C] (the reference's Mu(cfill of the liquid, Tfill)).  No
+ * boundary copy follows, as in the reference.  For a non-isothermal run the
+ * next step sees the temperature profile moved with the fields and later
+ * steps use temp_bottom with the frame offset increased by d.  The next
+ * gp_stats_get reports zero change; take the statistics of a step before
+ * shifting, as the reference's report does.
+ * No counterpart for the caller's decision (microsim_gp.c:115-126): shift by
+ * pos - Shiftj when pos > Shiftj, every 100 steps.
+ */
+int gp_check_shift(gp_ctx *ctx, long *pos);
+int gp_apply_shift_y(gp_ctx *ctx, long d, const double *mu_fill);
 
 const char *gp_strerror(int code);
 

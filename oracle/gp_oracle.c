@@ -50,6 +50,9 @@ typedef struct grid {
      temperature ghost rows hold their copies (1) or the linear profile (0) */
   double tb;
   int tproxy;
+  /* rows of a moving-frame shift right before this step (temperature moved with
+     the fields), 0 otherwise */
+  long tsh;
 } grid;
 
 #define AT(g, arr, a, x, y) (arr)[((size_t)(a) * (g)->rx + (x)) * (g)->ry + (y)]
@@ -122,14 +125,17 @@ static double temp_y(const grid *g, long y, int proxy) {
     if (y <= 2 && (tl == GP_BC_NEUMANN || tl == GP_BC_PERIODIC)) Y = p->bc_proxy_T[GP_FACE_YMIN][2 - y];
     if (y >= g->ry - 3 && (th == GP_BC_NEUMANN || th == GP_BC_PERIODIC)) Y = p->bc_proxy_T[GP_FACE_YMAX][y - (g->ry - 3)];
   }
+  /* after apply_shiftY (functions/shift.h:19) row Y holds the temperature of
+     row Y + tsh, except the top tsh rows */
+  if (Y < g->ry - g->tsh) Y += g->tsh;
   const double G = (p->tempgrad[1]) * p->dy / (p->tempgrad[2]);
   return g->tb + Y * G;
 }
 
-/* temp_bottom after t steps (microsim_gp.c:78-80,109-111; no moving frame) */
-static double temp_bottom(const gp_params *p, long t) {
+/* temp_bottom after t steps with the frame moved by shift_OFFSET rows
+   (microsim_gp.c:78-80,109-111) */
+static double temp_bottom(const gp_params *p, long shift_OFFSET, long t) {
   const double G = (p->tempgrad[1]) * p->dy / (p->tempgrad[2]);
-  const long shift_OFFSET = 0;
   double base = (p->tempgrad[3] / p->dy) - shift_OFFSET;
   if (t > 0) base = (p->tempgrad[3] / p->dy) - shift_OFFSET + ((p->tempgrad[4] / p->dy) * (t * p->dt));
   return p->tempgrad[0] - base * G;
@@ -901,8 +907,9 @@ int gpo_dphi(const gp_params *p, const double *phi, const double *mu, double *dp
   if (rc) return rc;
   grid G = {p, p->mesh_x + 6, p->mesh_y + 6, 0, p->nphases, p->ncomp};
   G.n = (size_t)G.rx * G.ry;
-  G.tb = temp_bottom(p, 0);
+  G.tb = temp_bottom(p, 0, 0);
   G.tproxy = 1;
+  G.tsh = 0;
   work w;
   if (alloc_work(&G, &w)) { free_work(&w); return GP_E_NOMEM; }
   stage_gradients(&G, phi, &w);
@@ -916,6 +923,10 @@ int gpo_step(const gp_params *p, double *phi, double *mu, long nsteps, int nthre
   return gpo_step_from(p, phi, mu, 0, nsteps, nthreads);
 }
 int gpo_step_from(const gp_params *p, double *phi, double *mu, long t0, long nsteps, int nthreads) {
+  return gpo_step_frame(p, phi, mu, t0, nsteps, 0, 0, nthreads);
+}
+int gpo_step_frame(const gp_params *p, double *phi, double *mu, long t0, long nsteps, long shift_off, long shift_pend,
+                   int nthreads) {
   int rc = check(p);
   if (rc) return rc;
 #ifdef _OPENMP
@@ -935,7 +946,8 @@ int gpo_step_from(const gp_params *p, double *phi, double *mu, long t0, long nst
   for (long t = 0; t < nsteps; t++) {
     /* step t+1 sees the temperature set after step t; its phase sweep sees the
        ghost copies only in the first step (after the initial boundary copies) */
-    G.tb = temp_bottom(p, t0 + t);
+    G.tsh = t == 0 ? shift_pend : 0;
+    G.tb = temp_bottom(p, shift_off - G.tsh, t0 + t);
     G.tproxy = t0 + t == 0;
     memcpy(phi_o, phi, sizeof(double) * G.n * G.P);
     memcpy(mu_o, mu, sizeof(double) * G.n * G.C);
@@ -991,5 +1003,40 @@ int gpo_smooth(const gp_params *p, double *phi, double *mu, long nsweeps, int nt
 #ifdef _OPENMP
   omp_set_num_threads(saved);
 #endif
+  return GP_OK;
+}
+
+/* check_SHIFT (functions/shift.h:44-66) folded over the interior rows
+   x in [3, MESH_X+2] (the reference's fold, solverloop/solverloop.h:61-66,
+   also visits the ghost rows 1, 2 and MESH_X+3 in the state of its sweep
+   before the boundary copy) */
+long gpo_check_shift(const gp_params *p, const double *phi) {
+  const long rx = p->mesh_x + 6, ry = p->mesh_y + 6, L = p->nphases - 1;
+  long best = 0;
+  for (long x = 3; x <= p->mesh_x + 2; x++) {
+    const double *col = phi + ((size_t)L * rx + x) * ry;
+    for (long y = 1; y <= ry - 1; y++)
+      if ((col[y - 1] - (1.0 - col[y - 1]) < 0.0) && (col[y] - (1.0 - col[y]) > 0.0) && y > best) best = y;
+  }
+  return best;
+}
+
+/* apply_shiftY (functions/shift.h:4-38) by d rows, in place: each padded
+   column moves d rows towards y = 0 and the top d rows become the last phase
+   at mu_fill */
+int gpo_apply_shift_y(const gp_params *p, double *phi, double *mu, long d, const double *mu_fill) {
+  const long rx = p->mesh_x + 6, ry = p->mesh_y + 6;
+  if (d < 1 || d > p->mesh_y) return GP_E_ARG;
+  for (long x = 0; x < rx; x++) {
+    for (long y = 0; y <= ry - 1 - d; y++) {
+      for (int b = 0; b < p->nphases; b++) phi[((size_t)b * rx + x) * ry + y] = phi[((size_t)b * rx + x) * ry + y + d];
+      for (int k = 0; k < p->ncomp; k++) mu[((size_t)k * rx + x) * ry + y] = mu[((size_t)k * rx + x) * ry + y + d];
+    }
+    for (long y = ry - d; y <= ry - 1; y++) {
+      for (int b = 0; b < p->nphases - 1; b++) phi[((size_t)b * rx + x) * ry + y] = 0.0;
+      phi[((size_t)(p->nphases - 1) * rx + x) * ry + y] = 1.0;
+      for (int k = 0; k < p->ncomp; k++) mu[((size_t)k * rx + x) * ry + y] = mu_fill[k];
+    }
+  }
   return GP_OK;
 }

@@ -30,6 +30,16 @@ int gpo_step(const gp_params *p, double *phi, double *mu, long nsteps, int nthre
 /* The same, continuing a run after t0 steps (the temperature of a
    non-isothermal run depends on the time step). */
 int gpo_step_from(const gp_params *p, double *phi, double *mu, long t0, long nsteps, int nthreads);
+/* The same with a moving frame: the frame has moved shift_off rows since the
+   run's start, the last shift_pend of them right before this call (the first
+   step then sees the temperature profile moved with the fields,
+   functions/shift.h:19; microsim_gp.c:108-126). */
+int gpo_step_frame(const gp_params *p, double *phi, double *mu, long t0, long nsteps, long shift_off, long shift_pend,
+                   int nthreads);
+/* Interface position of the moving frame (check_SHIFT over the interior rows). */
+long gpo_check_shift(const gp_params *p, const double *phi);
+/* apply_shiftY by d rows in place; new top rows are the last phase at mu_fill. */
+int gpo_apply_shift_y(const gp_params *p, double *phi, double *mu, long d, const double *mu_fill);
 
 /* Delta-phi of one phase sweep on the ring-extended region x,y in [2, rows-3]
    (zero elsewhere), for tests (calculate_divergence_phasefield.h:4-62). */

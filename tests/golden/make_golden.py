@@ -246,6 +246,12 @@ DRIVER_CASES = [
     dict(name="driver_p2c1_tempgrady_ascii", P=2, C=1, mesh=(40, 48), fmt="ASCII", steps=40, saveT=20,
          tempgrady="{0.96, 0.06, 80.0, 5, 1.6}", composition=1, restart=(20, 20),
          fill=["FILLCYLINDER = {0, 20, 16, 0, 0, 7};"]),
+    # moving frame (Shift, checked every 100 steps) over a planar front
+    dict(name="driver_p2c1_shift_binary", P=2, C=1, mesh=(24, 40), fmt="BINARY", steps=300, saveT=100,
+         shift=11, fill=["FILLCUBE = {0, 0, 0, 0, 23, 9, 0};"]),
+    dict(name="driver_p2c1_tempgrady_shift_ascii", P=2, C=1, mesh=(24, 40), fmt="ASCII", steps=300, saveT=100,
+         tempgrady="{0.96, 0.06, 80.0, 5, 1.6}", composition=1, shift=11, restart=(200, 100),
+         fill=["FILLCUBE = {0, 0, 0, 0, 23, 9, 0};"]),
     dict(name="driver_p3c2_tempgrady_binary", P=3, C=2, mesh=(32, 40), fmt="BINARY", steps=30, saveT=15,
          solver="TERNARY", tempgrady="{0.94, 0.1, 60.0, 2, 2.0}", composition=1,
          fill=["FILLCYLINDER = {0, 10, 12, 0, 0, 5};", "FILLCYLINDER = {1, 22, 26, 0, 0, 6};"]),
@@ -261,6 +267,9 @@ def run_driver_case(c, ref):
         keys["Gamma_abc"] = "{" + ",".join([str(c["gabc"])] * (P * (P - 1) * (P - 2) // 6)) + "}"
     if c.get("composition"):
         keys["Writecomposition"] = "1"
+    if c.get("shift") is not None:
+        keys["Shift"] = "1"
+        keys["Shiftj"] = str(c["shift"])
     extra = synth.thermo_lines(P, C, solver=c.get("solver", "BINARY")) + synth.bc_lines()
     if c.get("tempgrady"):
         keys["ISOTHERMAL"] = "0"

@@ -29,6 +29,13 @@ def oracle():
         L.gpo_step.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p, ct.c_long, ct.c_int]
         L.gpo_dphi.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.gpo_smooth.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p, ct.c_long, ct.c_int]
+        L.gpo_step_frame.restype = ct.c_int
+        L.gpo_step_frame.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p, ct.c_long, ct.c_long,
+                                     ct.c_long, ct.c_long, ct.c_int]
+        L.gpo_check_shift.restype = ct.c_long
+        L.gpo_check_shift.argtypes = [ct.POINTER(GpParams), ct.c_void_p]
+        L.gpo_apply_shift_y.restype = ct.c_int
+        L.gpo_apply_shift_y.argtypes = [ct.POINTER(GpParams), ct.c_void_p, ct.c_void_p, ct.c_long, ct.c_void_p]
         _orc = L
     return _orc
 
@@ -55,6 +62,51 @@ def oracle_run(p, phi, mu, nsteps, nthreads=0, t0=0):
     rc = oracle().gpo_step_from(ct.byref(p), _p(php), _p(mup), int(t0), int(nsteps), int(nthreads))
     assert rc == 0, rc
     return php[:, 3:-3, 3:-3].copy(), mup[:, 3:-3, 3:-3].copy()
+
+
+def liquid_fill_mu(inp):
+    """mu of the rows entering a moving frame: Mu(cfill of the liquid, Tfill) of
+    the liquid (functions/shift.h:24-36, function_F_01_Mu of functionF_01.h:19-29),
+    evaluated in binary64 in the reference's order."""
+    p = inp.p
+    L, C = p.nphases - 1, p.ncomp
+    c = [p.cfill[L][L][k] for k in range(C)]
+    out = []
+    for i in range(C):
+        s = 2.0 * p.A[L][i][i] * c[i] + (p.Beq[L][i] + p.dBbdT[L][i] * (inp.Tfill - p.Teq))
+        for j in range(C):
+            if i != j:
+                s += p.A[L][i][j] * c[j]
+        out.append(s)
+    return np.array(out)
+
+
+def oracle_frame_run(inp, phi, mu, nsteps, shiftj, save_every, nthreads=0):
+    """The driver's time loop with the moving frame on the oracle
+    (microsim_gp.c:101-138): after every 100th step, check_SHIFT and, when the
+    interface is above Shiftj, apply_shiftY by the difference.  Returns
+    {t: (phi, mu, frame offset, rows shifted right after step t)} at the
+    multiples of save_every."""
+    p = inp.p
+    L = oracle()
+    php, mup = pad(p, phi, mu)
+    fill = liquid_fill_mu(inp)
+    t = off = pend = 0
+    out = {}
+    while t < nsteps:
+        nxt = min(nsteps, (t // 100 + 1) * 100, (t // save_every + 1) * save_every)
+        assert L.gpo_step_frame(ct.byref(p), _p(php), _p(mup), t, nxt - t, off, pend, int(nthreads)) == 0
+        t, pend, shifted = nxt, 0, 0
+        if t % 100 == 0:
+            pos = L.gpo_check_shift(ct.byref(p), _p(php))
+            if pos > shiftj:
+                shifted = pos - shiftj
+                assert L.gpo_apply_shift_y(ct.byref(p), _p(php), _p(mup), shifted, _p(fill)) == 0
+                off += shifted
+                pend = shifted
+        if t % save_every == 0:
+            out[t] = (php[:, 3:-3, 3:-3].copy(), mup[:, 3:-3, 3:-3].copy(), off, shifted)
+    return out
 
 
 def oracle_smooth(p, phi, mu, nsweeps, nthreads=0):

@@ -253,3 +253,91 @@ def test_gpu_f32_precision_report_multigrain():
     assert max(e["rel_l2_err"] for e in r["mu"]) < 2e-3
     assert r["phi_cells_above_1e-4"] < 0.05 * 256 * 256
     assert r["solute_f32_minus_f64_max"] < 1e-7
+
+
+def _solver_frame_run(s, nsteps, shiftj, save_every, fill):
+    """The driver's loop with the moving frame on a Solver (see gputil.oracle_frame_run)."""
+    t, out, positions = 0, {}, []
+    while t < nsteps:
+        nxt = min(nsteps, (t // 100 + 1) * 100, (t // save_every + 1) * save_every)
+        s.step(nxt - t)
+        t, shifted = nxt, 0
+        if t % 100 == 0:
+            pos = s.check_shift()
+            positions.append(pos)
+            if pos > shiftj:
+                shifted = pos - shiftj
+                s.apply_shift_y(shifted, fill)
+        if t % save_every == 0:
+            phi, mu = s.get_state()
+            out[t] = (phi, mu, shifted)
+    return out, positions
+
+
+def _frame_case(kind):
+    """(input, phi0, mu0, shiftj): a planar two-solid front at the bottom of a
+    liquid, isothermal or in a moving temperature gradient."""
+    if kind == "iso_p3c2":
+        inp = gputil.make_params(P=3, C=2, mesh=(96, 128), solver="TERNARY")
+    else:
+        inp = gputil.make_params(P=3, C=2, mesh=(96, 128), solver="TERNARY", tempgrady="{0.94, 0.1, 60.0, 2, 2.0}")
+    p = inp.p
+    MX, MY = p.mesh_x, p.mesh_y
+    x = np.arange(MX)[:, None]
+    y = np.arange(MY)[None, :]
+    front = 20 + 3 * np.sin(2 * np.pi * x / 48.0)
+    solid = y < front
+    phi = np.zeros((3, MX, MY))
+    phi[0] = solid & (x < MX // 2)
+    phi[1] = solid & (x >= MX // 2)
+    phi[2] = ~solid
+    mu = np.broadcast_to(gputil.liquid_fill_mu(inp)[:, None, None], (2, MX, MY)).copy()
+    return inp, phi, mu, 21
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,path", [("iso_p3c2", 1), ("iso_p3c2", 0), ("iso_p3c2", 3), ("ni_p3c2", 1)])
+def test_gpu_moving_frame_matches_oracle(kind, path):
+    """Moving frame (Shift): check_SHIFT and apply_shiftY every 100 steps over
+    400 steps of a 96x128 ternary front; binary64 states equal the oracle's
+    after every step block, frame positions included."""
+    print(f"\n[gpu] moving frame {kind} path={path} 96x128 400 steps", flush=True)
+    inp, phi0, mu0, shiftj = _frame_case(kind)
+    ref = gputil.oracle_frame_run(inp, phi0, mu0, 400, shiftj, 100)
+    s = gpstep.Solver(inp, "f64", path=path)
+    s.set_state(phi0, mu0)
+    got, positions = _solver_frame_run(s, 400, shiftj, 100, gputil.liquid_fill_mu(inp))
+    print(f"  frame positions {positions}, shifts {[got[t][2] for t in sorted(got)]}", flush=True)
+    assert sum(got[t][2] for t in got) > 0
+    for t in sorted(ref):
+        phi_r, mu_r, off, shifted = ref[t]
+        phi, mu, sh = got[t]
+        assert sh == shifted, (t, sh, shifted)
+        ep, em = _cmp(phi, phi_r)[0], _cmp(mu, mu_r)[0]
+        assert ep <= TOL64 and em <= TOL64, (t, ep, em)
+
+
+@pytest.mark.gpu
+def test_gpu_moving_frame_f32_and_arguments():
+    """The binary32 frame takes the same shifts as binary64 and stays close;
+    shifts outside 1..MESH_Y and non-finite fill values are refused."""
+    print("\n[gpu] moving frame f32 vs f64, ni_p3c2 96x128 400 steps", flush=True)
+    inp, phi0, mu0, shiftj = _frame_case("ni_p3c2")
+    fill = gputil.liquid_fill_mu(inp)
+    res = {}
+    for dt in ("f64", "f32"):
+        s = gpstep.Solver(inp, dt)
+        s.set_state(phi0, mu0)
+        res[dt] = _solver_frame_run(s, 400, shiftj, 100, fill)
+    assert res["f32"][1] == res["f64"][1]
+    phi32, mu32, _ = res["f32"][0][400]
+    phi64, mu64, _ = res["f64"][0][400]
+    ephi = np.abs(phi32 - phi64).max()
+    emu = np.abs(mu32 - mu64).max() / max(np.ptp(mu64), 1e-30)
+    print(f"  max|dphi|={ephi:.3e} rel max|dmu|={emu:.3e}", flush=True)
+    assert ephi < 1e-4 and emu < 1e-4
+    for bad in (0, -1, inp.p.mesh_y + 1):
+        with pytest.raises(gpstep.GPError):
+            s.apply_shift_y(bad, fill)
+    with pytest.raises(gpstep.GPError):
+        s.apply_shift_y(1, np.full(2, np.nan))

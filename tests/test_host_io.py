@@ -32,8 +32,9 @@ def _blocks(text):
     return out
 
 
-@pytest.mark.parametrize("case", gputil.driver_cases())
+@pytest.mark.parametrize("case", [c for c in gputil.driver_cases() if "shift" not in c])
 def test_vtk_writer_reproduces_reference_files(case, tmp_path):
+    """(Runs with a moving frame are covered with their frame offsets below.)"""
     z = np.load(os.path.join(gputil.GOLDEN, case + ".npz"))
     first = gpstep.read_input_text(str(z["infile"]), str(tmp_path))
     restart = None
@@ -89,7 +90,52 @@ def test_temp_bottom_is_the_reference_expression():
         p = gpstep.read_input_text(str(z["infile"]), d).p
     base_temp, DeltaT, Distance, offset, velocity = (p.tempgrad[i] for i in range(5))
     G = DeltaT * p.dy / Distance
-    for t in (0, 1, 7, 100, 12345):
-        base = offset / p.dy - 0 + ((velocity / p.dy) * (t * p.dt)) if t else offset / p.dy - 0
-        assert gpstep.temp_bottom(p, t) == base_temp - base * G
+    for S in (0, 3):
+        for t in (0, 1, 7, 100, 12345):
+            base = offset / p.dy - S + ((velocity / p.dy) * (t * p.dt)) if t else offset / p.dy - S
+            assert gpstep.temp_bottom(p, t, S) == base_temp - base * G
     assert gpstep.temp_bottom(p, 1) != gpstep.temp_bottom(p, 0)
+
+
+@pytest.mark.parametrize("case", [c for c in gputil.driver_cases() if "shift" in c])
+def test_oracle_moving_frame_reproduces_reference_files(case, tmp_path):
+    """The oracle's moving frame (check_SHIFT over the interior rows, apply_shiftY,
+    the temperature of the step after a shift, the restart's frame position from
+    DATA/shift.dat) reproduces every file of a reference run with Shift = 1,
+    through the library's VTK writer: binary files byte for byte, text files
+    line for line."""
+    z = np.load(os.path.join(gputil.GOLDEN, case + ".npz"))
+    (tmp_path / "Filling.in").write_text(str(z["filling"]))
+    inp = gpstep.read_input_text(str(z["infile"]), str(tmp_path))
+    assert inp.shift == 1
+    phi0, mu0 = gpstep.fill_domain(inp, str(tmp_path / "Filling.in"))
+    save = int(re.search(r"SAVET\s*=\s*(\d+)", str(z["infile"])).group(1))
+    nsteps = int(re.search(r"NTIMESTEPS\s*=\s*(\d+)", str(z["infile"])).group(1))
+    shift_lines = bytes(z["file_shift.dat"]).decode().split("\n")
+    runs = [(inp, gputil.oracle_frame_run(inp, phi0, mu0, nsteps, inp.shiftj, save), 0, 0)]
+    if "infile_restart" in z:
+        (tmp_path / "r").mkdir()
+        rin = gpstep.read_input_text(str(z["infile_restart"]), str(tmp_path / "r"))
+        t0 = rin.starttime
+        # the reference reads STARTTIME/SAVET + 1 lines of DATA/shift.dat
+        position = int(shift_lines[t0 // save].split()[1])
+        p = rin.p
+        if not p.isothermal:
+            p.tempgrad[3] = p.tempgrad[3] + math.floor(p.tempgrad[4] * (t0 * p.dt)) - position * p.dy
+        src = tmp_path / "restart.vtk"
+        src.write_bytes(bytes(z[f"file_out_{t0}.vtk"]))
+        phi, mu = gpstep.vtk_read(str(src), rin, not rin.ascii)
+        runs.append((rin, gputil.oracle_frame_run(rin, phi, mu, rin.ntimesteps, rin.shiftj, save), t0, position))
+    # the files after a restart are those of the second run
+    want_files, lines = {}, []
+    for k, (ri, res, t0, position) in enumerate(runs):
+        for t, (phi, mu, off, shifted) in sorted(res.items()):
+            want_files[t + t0] = (ri, phi, mu, t, off, shifted)
+            lines.append(f"{t + t0} {off + position}")
+    assert lines == [ln for ln in shift_lines if ln]
+    for T, (ri, phi, mu, t, off, shifted) in sorted(want_files.items()):
+        out = tmp_path / "out.vtk"
+        gpstep.vtk_write(str(out), ri, phi, mu, not ri.ascii, t, off, shifted)
+        got, want = out.read_bytes(), bytes(z[f"file_out_{T}.vtk"])
+        assert got == want, f"{case}: out_{T}.vtk differs"
+    assert any(s for (_, res, _, _) in runs for (_, _, _, s) in res.values()), "no shift in this fixture"
