@@ -78,8 +78,11 @@ def _check_report(ours_text, ref_text, z):
     # interior extrema of the report steps only, a subset.  Likewise the
     # squared change is summed over interior cells only.  So the driver's
     # Max <= reference Max, Min >= reference Min, change <= reference change,
-    # with equality for the phase fractions, which stay in [0, 1].
+    # with equality for the phase fractions, which stay in [0, 1].  With a
+    # short report interval the folded extrema of mu also stay within 5% of the
+    # reference's; over 100 steps they can miss a dip between report steps.
     nphase = int(re.search(r"NUMPHASES\s*=\s*(\d+)", str(z["infile"])).group(1))
+    track = int(re.search(r"TRACK_PROGRESS\s*=\s*(\d+)", str(z["infile"])).group(1))
     for (t0, rows0), (t1, rows1) in zip(ours, ref):
         assert t0 == t1
         assert [x[0] for x in rows0] == [x[0] for x in rows1]
@@ -88,6 +91,7 @@ def _check_report(ours_text, ref_text, z):
                 assert a[1:3] == b[1:3]
             else:
                 assert float(a[1]) <= float(b[1]) and float(a[2]) >= float(b[2])
-                assert float(a[2]) - float(b[2]) <= 0.05 * abs(float(b[2]))
+                if track <= 20:
+                    assert float(a[2]) - float(b[2]) <= 0.05 * abs(float(b[2]))
             assert a[3] <= b[3] * (1 + 1e-9) + 1e-300
             assert a[3] >= 0.5 * b[3]

@@ -279,24 +279,30 @@ def _frame_case(kind):
     liquid, isothermal or in a moving temperature gradient."""
     if kind == "iso_p3c2":
         inp = gputil.make_params(P=3, C=2, mesh=(96, 128), solver="TERNARY")
-    else:
+    elif kind == "ni_p3c2":
         inp = gputil.make_params(P=3, C=2, mesh=(96, 128), solver="TERNARY", tempgrady="{0.94, 0.1, 60.0, 2, 2.0}")
+    else:
+        inp = gputil.make_params(P=2, C=1, mesh=(96, 128), tempgrady="{0.96, 0.06, 80.0, 5, 1.6}")
     p = inp.p
+    P, C = p.nphases, p.ncomp
     MX, MY = p.mesh_x, p.mesh_y
     x = np.arange(MX)[:, None]
     y = np.arange(MY)[None, :]
     front = 20 + 3 * np.sin(2 * np.pi * x / 48.0)
     solid = y < front
-    phi = np.zeros((3, MX, MY))
-    phi[0] = solid & (x < MX // 2)
-    phi[1] = solid & (x >= MX // 2)
-    phi[2] = ~solid
-    mu = np.broadcast_to(gputil.liquid_fill_mu(inp)[:, None, None], (2, MX, MY)).copy()
+    phi = np.zeros((P, MX, MY))
+    if P == 3:
+        phi[0] = solid & (x < MX // 2)
+        phi[1] = solid & (x >= MX // 2)
+    else:
+        phi[0] = solid
+    phi[P - 1] = ~solid
+    mu = np.broadcast_to(gputil.liquid_fill_mu(inp)[:, None, None], (C, MX, MY)).copy()
     return inp, phi, mu, 21
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind,path", [("iso_p3c2", 1), ("iso_p3c2", 0), ("iso_p3c2", 3), ("ni_p3c2", 1)])
+@pytest.mark.parametrize("kind,path", [("iso_p3c2", 1), ("iso_p3c2", 0), ("iso_p3c2", 3), ("ni_p3c2", 1), ("ni_p2c1", 1)])
 def test_gpu_moving_frame_matches_oracle(kind, path):
     """Moving frame (Shift): check_SHIFT and apply_shiftY every 100 steps over
     400 steps of a 96x128 ternary front; binary64 states equal the oracle's
@@ -319,10 +325,12 @@ def test_gpu_moving_frame_matches_oracle(kind, path):
 
 @pytest.mark.gpu
 def test_gpu_moving_frame_f32_and_arguments():
-    """The binary32 frame takes the same shifts as binary64 and stays close;
-    shifts outside 1..MESH_Y and non-finite fill values are refused."""
-    print("\n[gpu] moving frame f32 vs f64, ni_p3c2 96x128 400 steps", flush=True)
-    inp, phi0, mu0, shiftj = _frame_case("ni_p3c2")
+    """The binary32 frame takes the same shifts as binary64 and stays close on a
+    two-phase front (a triple junction at the front makes the two precisions
+    part pointwise, with or without a frame); shifts outside 1..MESH_Y and
+    non-finite fill values are refused."""
+    print("\n[gpu] moving frame f32 vs f64, ni_p2c1 96x128 400 steps", flush=True)
+    inp, phi0, mu0, shiftj = _frame_case("ni_p2c1")
     fill = gputil.liquid_fill_mu(inp)
     res = {}
     for dt in ("f64", "f32"):
@@ -340,4 +348,4 @@ def test_gpu_moving_frame_f32_and_arguments():
         with pytest.raises(gpstep.GPError):
             s.apply_shift_y(bad, fill)
     with pytest.raises(gpstep.GPError):
-        s.apply_shift_y(1, np.full(2, np.nan))
+        s.apply_shift_y(1, np.full(inp.p.ncomp, np.nan))
