@@ -19,7 +19,8 @@ struct gp_ctx {
   void *phi[2], *mu[2]; // ping-pong field buffers (padded planes)
   void *mulo[2];        // binary32 path: low-order part of mu (compensated update)
   void *dphi;           // two-pass path: delta-phi planes
-  signed char *dcode;   // sparse two-pass path: per-cell one-hot / zero-delta-phi code
+  signed char *dcode;   // sparse and split two-pass paths: per-cell one-hot / zero-delta-phi code
+  int *dsplit;          // split two-pass path: 2 counters, phase-sweep and mu-sweep block lists
   void *dprm;           // DevParams<R> on the device
   double *dpart;        // stats partials
   int nstat_blocks;
@@ -29,7 +30,7 @@ struct gp_ctx {
   long shift_off;       // rows the moving frame has moved since the state was loaded
   long shift_pend;      // rows of a shift not yet followed by a step (temperature of that step)
   int *drow;            // interface position per interior row (allocated on first use)
-  int path;             // 0 fused, 1 two-pass
+  int path;             // 0 sparse fused, 1 two-pass, 2 dense fused, 3 sparse two-pass, 4 split two-pass
   hipStream_t stream;
   hipEvent_t ev0, ev1;
   double last_ms;

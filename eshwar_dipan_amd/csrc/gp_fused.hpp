@@ -325,6 +325,83 @@ __device__ __forceinline__ bool phase_active(const DevParams<R>& p, const R* __r
   return any;
 }
 
+// Split two-pass step (path 4).  The dense two-pass kernels need about 240
+// registers per lane for interface cells, which leaves two waves per SIMD also
+// for bulk cells, whose few loads then go unhidden.  Each sweep of a split step
+// is a light kernel over the full grid, with the block shape of the dense
+// kernel, that finishes every cell with an exact shortcut -- phase sweep: no
+// active phase, delta-phi from the projection alone; mu sweep: bulk_cell,
+// decided on the codes of the 3x3 block -- and appends each block holding any
+// other cell to a list; the dense kernel then runs over the listed blocks only
+// and recomputes all their cells (the shortcuts are bit-identical to the full
+// arithmetic).  Counters: cnt[0] (phase list) is cleared by the mu-sweep light
+// kernel, cnt[1] (mu list) by the next phase-sweep light kernel, each after
+// the kernel that read it has finished.
+template <int P, class R>
+__global__ void __launch_bounds__(BX * BY) k_phase_split(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                                         R* __restrict__ dphi, signed char* __restrict__ code,
+                                                         int* __restrict__ list, int* __restrict__ cnt) {
+  const DevParams<R>& p = *prm;
+  int bx, by;
+  blk_xy(bx, by);
+  const int y = 2 + bx * BX + threadIdx.x;
+  const int x = 2 + by * BY + threadIdx.y;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) cnt[1] = 0;
+  int heavy = 0;
+  if (x <= p.rx - 3 && y <= p.ry - 3) {
+    R ph[P];
+    if (phase_active<P>(p, phi, x, y, ph)) {
+      heavy = 1;
+      code[(size_t)x * p.pitch + y] = -1;
+    } else {
+      R dp[P];
+      bool act[P];
+#pragma unroll
+      for (int a = 0; a < P; a++) {
+        dp[a] = R(0);
+        act[a] = false;
+      }
+      project_simplex<P>(ph, act, dp);
+      bool zero = true;
+#pragma unroll
+      for (int a = 0; a < P; a++) {
+        dphi[idx(p, a, x, y)] = dp[a];
+        zero = zero && dp[a] == R(0) && !signbit(dp[a]);
+      }
+      code[(size_t)x * p.pitch + y] = (signed char)(zero ? onehot_phase<P>(ph) : -1);
+    }
+  }
+  if (__syncthreads_or(heavy) && threadIdx.x == 0 && threadIdx.y == 0) list[atomicAdd(&cnt[0], 1)] = by * gridDim.x + bx;
+}
+
+// mu sweep of a split step: the cells (block shape of k_conc_lds / k_conc_march
+// with one row per wave) whose 3x3 block carries one code a >= 0 take bulk_cell.
+template <int P, int C, class R>
+__global__ void __launch_bounds__(BX * BY) k_conc_split(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
+                                                        const R* __restrict__ mu, const R* __restrict__ mulo,
+                                                        const signed char* __restrict__ code, R* __restrict__ phi_o,
+                                                        R* __restrict__ mu_o, R* __restrict__ mulo_o,
+                                                        int* __restrict__ list, int* __restrict__ cnt) {
+  const DevParams<R>& p = *prm;
+  int bx, by;
+  blk_xy(bx, by);
+  const int lane = threadIdx.x;
+  const int y = 2 + bx * MW + lane;  // lane 0 sits one column before the block's cells, as in k_conc_lds
+  const int x = 3 + by * BY + threadIdx.y;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) cnt[0] = 0;
+  int heavy = 0;
+  if (lane > 0 && x <= p.mesh_x + 2 && y <= p.mesh_y + 2) {
+    int cc = code[(size_t)x * p.pitch + y];
+#pragma unroll
+    for (int u = -1; u <= 1; u++)
+#pragma unroll
+      for (int w = -1; w <= 1; w++) cc = code[(size_t)(x + u) * p.pitch + y + w] == cc ? cc : -1;
+    if (cc >= 0) bulk_cell<P, C, R>(p, phi, mu, mulo, phi_o, mu_o, mulo_o, x, y);
+    else heavy = 1;
+  }
+  if (__syncthreads_or(heavy) && threadIdx.x == 0 && threadIdx.y == 0) list[atomicAdd(&cnt[1], 1)] = by * gridDim.x + bx;
+}
+
 // Sparse two-pass engine for interface tiles.  k_phase_tiles computes
 // delta-phi on each listed tile plus its 1-cell ring into the dphi planes;
 // k_conc_tiles then updates the tile.  Both kernels first sort the tile's cells

@@ -48,6 +48,22 @@ __device__ __forceinline__ void blk_xy(int& bx, int& by) {
   by = l / gridDim.x;
 }
 
+// Block of a list-driven launch (split two-pass step): block i of a 1-D grid
+// takes logical block list[i] of a gx-wide grid; blocks past *count return.
+// Without a list, the XCD-aware order of the full 2-D grid.
+__device__ __forceinline__ bool blk_pick(const int* __restrict__ list, const int* __restrict__ count, int gx, int& bx,
+                                         int& by) {
+  if (!list) {
+    blk_xy(bx, by);
+    return true;
+  }
+  if ((int)blockIdx.x >= *count) return false;
+  const int l = list[blockIdx.x];
+  bx = l % gx;
+  by = l / gx;
+  return true;
+}
+
 template <class R>
 __device__ __forceinline__ size_t idx(const DevParams<R>& p, int a, int x, int y) {
   return (size_t)a * p.plane + (size_t)x * p.pitch + y;
@@ -221,10 +237,11 @@ __host__ __device__ inline dim3 phase_grid(int rx, int ry) { return dim3((ry - 4
 template <int P, int C, class R, bool NI = false>
 __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_phase(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                    const R* __restrict__ mu, R* __restrict__ dphi, R tb = R(0),
-                                                   int tmode = 0) {
+                                                   int tmode = 0, const int* __restrict__ list = nullptr,
+                                                   const int* __restrict__ count = nullptr, int gx = 0) {
   const DevParams<R>& p = *prm;
   int bx, by;
-  blk_xy(bx, by);
+  if (!blk_pick(list, count, gx, bx, by)) return;
   const int y = 2 + bx * BX + threadIdx.x;
   const int x = 2 + by * BY + threadIdx.y;
   if (x > p.rx - 3 || y > p.ry - 3) return;
@@ -254,11 +271,13 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
                                                               const R* __restrict__ phi, const R* __restrict__ mu,
                                                               const R* __restrict__ dphi, const R* __restrict__ mulo,
                                                               R* __restrict__ phi_o, R* __restrict__ mu_o,
-                                                              R* __restrict__ mulo_o, R tb = R(0), int tsh = 0) {
+                                                              R* __restrict__ mulo_o, R tb = R(0), int tsh = 0,
+                                                              const int* __restrict__ list = nullptr,
+                                                              const int* __restrict__ count = nullptr, int gx = 0) {
   const DevParams<R>& p = *prm;
   const int lane = threadIdx.x;
   int bx, by;
-  blk_xy(bx, by);
+  if (!blk_pick(list, count, gx, bx, by)) return;
   const int y = 2 + bx * MW + lane;  // lane 0 sits one column before the wave's cells
   const int yc = min(y, p.mesh_y + 2);
   // temperatures of the cell row and of the row above (the sweep runs after the boundary copies)
@@ -345,12 +364,14 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
                                                             const R* __restrict__ phi, const R* __restrict__ mu,
                                                             const R* __restrict__ dphi, const R* __restrict__ mulo,
                                                             R* __restrict__ phi_o, R* __restrict__ mu_o,
-                                                            R* __restrict__ mulo_o, R tb = R(0), int tsh = 0) {
+                                                            R* __restrict__ mulo_o, R tb = R(0), int tsh = 0,
+                                                            const int* __restrict__ list = nullptr,
+                                                            const int* __restrict__ count = nullptr, int gx = 0) {
   __shared__ R jx[C][BY][BX];
   const DevParams<R>& p = *prm;
   const int lane = threadIdx.x, w = threadIdx.y;
   int bx, by;
-  blk_xy(bx, by);
+  if (!blk_pick(list, count, gx, bx, by)) return;  // uniform over the block
   const int y = 2 + bx * MW + lane;  // lane 0 sits one column before the wave's cells
   const int yc = min(y, p.mesh_y + 2);
   // temperatures of the cell row and of the row above (the sweep runs after the boundary copies)

@@ -210,8 +210,44 @@ int run_sparse(gp_ctx *c, long nsteps) {
 // 109-111)
 inline double temp_bottom(const gp_params &h, long shift_off, long t) { return gp_temp_bottom(&h, shift_off, t); }
 
+// Split two-pass steps (path 4, isothermal): per sweep a light kernel over the
+// grid that finishes the cells with an exact shortcut and lists the blocks
+// that hold others, then the dense kernel over the listed blocks only.  The
+// dense kernels take a 1-D grid of every block; blocks past the list's count
+// return at once.
+template <int P, int C, class R>
+int run_split(gp_ctx *c, long nsteps) {
+  static_assert(KM == 1, "the split mu sweep assumes one row per wave");
+  const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
+  const dim3 blk(BX, BY);
+  const dim3 gp = phase_grid(c->rx, c->ry);
+  const dim3 gc(((int)c->hp.mesh_y + MW - 1) / MW, ((int)c->hp.mesh_x + BY - 1) / BY);
+  int *cnt = c->dsplit, *lp = cnt + 2, *lc = lp + gp.x * gp.y;
+  for (long t = 0; t < nsteps; t++) {
+    const R *phi = (const R *)c->phi[c->cur], *mu = (const R *)c->mu[c->cur], *mulo = (const R *)c->mulo[c->cur];
+    R *phi_o = (R *)c->phi[c->cur ^ 1], *mu_o = (R *)c->mu[c->cur ^ 1], *mulo_o = (R *)c->mulo[c->cur ^ 1];
+    R *dphi = (R *)c->dphi;
+    hipLaunchKernelGGL((k_phase_split<P, R>), gp, blk, 0, c->stream, prm, phi, dphi, c->dcode, lp, cnt);
+    hipLaunchKernelGGL((k_phase<P, C, R>), dim3(gp.x * gp.y), blk, 0, c->stream, prm, phi, mu, dphi, R(0), 0,
+                       (const int *)lp, (const int *)cnt, (int)gp.x);
+    hipLaunchKernelGGL((k_conc_split<P, C, R>), gc, blk, 0, c->stream, prm, phi, mu, mulo, (const signed char *)c->dcode,
+                       phi_o, mu_o, mulo_o, lc, cnt);
+    if constexpr (Ops<R>::Exact)
+      hipLaunchKernelGGL((k_conc_lds<P, C, R>), dim3(gc.x * gc.y), blk, 0, c->stream, prm, phi, mu, (const R *)dphi,
+                         mulo, phi_o, mu_o, mulo_o, R(0), 0, (const int *)lc, (const int *)(cnt + 1), (int)gc.x);
+    else
+      hipLaunchKernelGGL((k_conc_march<P, C, R>), dim3(gc.x * gc.y), blk, 0, c->stream, prm, phi, mu, (const R *)dphi,
+                         mulo, phi_o, mu_o, mulo_o, R(0), 0, (const int *)lc, (const int *)(cnt + 1), (int)gc.x);
+    c->cur ^= 1;
+  }
+  c->flags_valid = 0;
+  GP_HIP_CHECK(hipGetLastError());
+  return GP_OK;
+}
+
 template <int P, int C, class R>
 int run_steps(gp_ctx *c, long nsteps) {
+  if (c->path == 4) return run_split<P, C, R>(c, nsteps);
   if (c->path == 0 || c->path == 2 || c->path == 3) {
     // the fused kernel (paths 0, 2) and ensembles of grids use their own constraints
     int s = (c->path == 3 && shape_allowed<P>(c->shape)) ? c->shape : 0;

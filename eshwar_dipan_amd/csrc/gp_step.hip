@@ -192,6 +192,7 @@ static void free_ctx(gp_ctx *c) {
   }
   if (c->dphi) (void)hipFree(c->dphi);
   if (c->dcode) (void)hipFree(c->dcode);
+  if (c->dsplit) (void)hipFree(c->dsplit);
   if (c->dprm) (void)hipFree(c->dprm);
   if (c->dpart) (void)hipFree(c->dpart);
   if (c->drow) (void)hipFree(c->drow);
@@ -247,6 +248,13 @@ static int ctx_create(const gp_params *p, int dtype, bool own_stream, gp_ctx **o
   ok = ok && hipMalloc(&c->dphi, pb * c->P) == hipSuccess && hipMemset(c->dphi, 0, pb * c->P) == hipSuccess;
   ok = ok && hipMalloc((void **)&c->dcode, (size_t)c->plane) == hipSuccess &&
        hipMemset(c->dcode, 0xff, (size_t)c->plane) == hipSuccess;
+  {
+    const dim3 gp = gpk::phase_grid(c->rx, c->ry);
+    const size_t nsplit = 2 + (size_t)gp.x * gp.y + (size_t)((p->mesh_y + gpk::MW - 1) / gpk::MW) *
+                                                        ((p->mesh_x + gpk::BY - 1) / gpk::BY);
+    ok = ok && hipMalloc((void **)&c->dsplit, sizeof(int) * nsplit) == hipSuccess &&
+         hipMemset(c->dsplit, 0, sizeof(int) * nsplit) == hipSuccess;
+  }
   long ncell = p->mesh_x * p->mesh_y;
   c->nstat_blocks = (int)std::min<long>(1024, (ncell + 255) / 256);
   int nv = 3 * c->P + 3 * c->C + 1 + c->P + c->C;
@@ -312,7 +320,7 @@ int gp_set_graphs(gp_ctx *c, int on) {
 }
 
 int gp_set_path(gp_ctx *c, int path) {
-  if (!c || path < -1 || path > 3 || (c->nmember && path != 3 && path != -1)) return GP_E_ARG;
+  if (!c || path < -1 || path > 4 || (c->nmember && path != 3 && path != -1)) return GP_E_ARG;
   if (!c->hp.isothermal) {  // non-isothermal runs step on the two-pass path only
     if (path != 1 && path != -1) return GP_E_UNSUPPORTED;
     c->auto_mode = 0;
@@ -422,7 +430,7 @@ static int run_block(gp_ctx *c) {
   const int gi = graph_slot(c);
   if ((rc = capture_graph(c, gi))) return rc;
   if (hipGraphLaunch(c->gcache[gi][c->cur], c->stream) != hipSuccess) return GP_E_HIP;
-  if (c->path == 1) c->flags_valid = 0;  // the two-pass kernels keep no tile flags
+  if (c->path == 1 || c->path == 4) c->flags_valid = 0;  // the two-pass kernels keep no tile flags
   return GP_OK;
 }
 
@@ -430,8 +438,8 @@ static int run_block(gp_ctx *c) {
 // run), then the fastest is kept until the next re-timing.  The candidates share
 // the default tile set, so no graph is invalidated by the switches.
 static int retune(gp_ctx *c, long *left) {
-  const int cand[3] = {1, 0, 3};
-  const int ncand = c->nmember ? 1 : 3;
+  const int cand[4] = {1, 4, 0, 3};
+  const int ncand = c->nmember ? 1 : 4;
   double best = 1e300;
   int pick = c->path;
   hipEvent_t a = c->ev0, b = c->ev1;
@@ -455,7 +463,7 @@ static int retune(gp_ctx *c, long *left) {
       rc = GP_E_HIP;
       break;
     }
-    if (c->path == 1) c->flags_valid = 0;
+    if (c->path == 1 || c->path == 4) c->flags_valid = 0;
     float ms = 0;
     (void)hipEventElapsedTime(&ms, t0, t1);
     if (ms < best) {
@@ -546,13 +554,14 @@ int gp_tile_shape(gp_ctx *c, int *tile_x, int *tile_y) {
 // candidates give bit-identical binary64 results.
 static int autotune(gp_ctx *c, long steps, bool paths) {
   struct Cand { int path, shape; };
-  Cand cand[gpk::kNumShapes + 2];
+  Cand cand[gpk::kNumShapes + 3];
   int n = 0;
   for (int s = 0; s < gpk::kNumShapes; s++)
     if (s == 0 || c->P <= 4) cand[n++] = {3, s};
   if (paths) {
     cand[n++] = {0, 0};
     cand[n++] = {1, 0};
+    cand[n++] = {4, 0};
   }
   double best = 1e300;
   c->auto_mode = 0;
