@@ -74,7 +74,7 @@ def adjoint_report(a):
     mismatch to a target made with perturbed (dab, Tau, GAMMA, D_liquid), by the
     adjoint with all states kept and with 20 snapshots, against central
     differences (h = 1e-6 relative); time of the adjoint run over the fastest
-    forward run of the same steps (automatic, two-pass or sparse fused path)."""
+    forward run of the same steps (automatic, two-pass, split two-pass or sparse fused path)."""
     import copy
     import numpy as np
     from eshwar_dipan_amd import gpstep, workloads
@@ -114,9 +114,10 @@ def adjoint_report(a):
            "config": {"workload": desc, "mesh_x": p.mesh_x, "mesh_y": p.mesh_y}, "runs": []}
     for dtype in ("f64", "f32"):
         # forward time-to-solution of the same run, fastest of the automatic,
-        # two-pass and sparse fused paths (graphs captured outside the timing)
+        # two-pass, split two-pass and sparse fused paths (graphs captured
+        # outside the timing)
         t_paths = {}
-        for path in (-1, 1, 0):
+        for path in (-1, 1, 4, 0):
             s = gpstep.Solver(p, dtype, path=path)
             s.set_state(phi0, mu0)
             s.step(64)
@@ -139,7 +140,7 @@ def adjoint_report(a):
             out["runs"].append({"dtype": dtype, "snapshots": nsnap or steps, "loss": loss, "grad": grad,
                                 "forward_s": t_fwd, "adjoint_s": t_adj, "ratio": t_adj / t_fwd,
                                 "forward_s_by_path": {"automatic": t_paths[-1], "two-pass": t_paths[1],
-                                                      "sparse fused": t_paths[0]},
+                                                      "split two-pass": t_paths[4], "sparse fused": t_paths[0]},
                                 "adjoint_forward_steps": nfwd, "device_fwd_ms": f_ms, "device_rev_ms": r_ms})
             print(f"{dtype} nsnap={nsnap}: adjoint/forward {t_adj / t_fwd:.2f}", file=sys.stderr, flush=True)
         s.close()
@@ -250,7 +251,8 @@ def main():
     ap.add_argument("--workload", default=None, help="c1, c2 (default), c3, c4 (ensemble), c5")
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--path", type=int, default=-1,
-                    help="-1 automatic (default), 0 sparse fused, 1 two-pass, 2 dense fused, 3 sparse two-pass")
+                    help="-1 automatic (default), 0 sparse fused, 1 two-pass, 2 dense fused, 3 sparse two-pass, "
+                         "4 split two-pass")
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="ensemble: keep the default tile shape")
     ap.add_argument("--cpu-steps", type=int, default=None)
@@ -315,12 +317,13 @@ def main():
     wall = time.perf_counter() - t0
     dev_ms = s.last_step_ms()
     names = ["sparse: fused LDS kernel on interface tiles + bulk kernel", "two-pass", "dense fused",
-             "sparse: two-pass kernels on interface tiles + bulk kernel"]
+             "sparse: two-pass kernels on interface tiles + bulk kernel",
+             "split two-pass: light kernels finish bulk cells, two-pass kernels on the listed interface blocks"]
     path_last, auto = s.path()
-    path_desc = (f"automatic (one timed 32-step block of each of two-pass, sparse fused, sparse two-pass at the start, "
+    path_desc = (f"automatic (one timed 32-step block of each of two-pass, split two-pass, sparse fused, sparse two-pass at the start, "
                  f"after 64 blocks, then at gaps doubling up to 512 blocks while the choice holds; "
                  f"last: {names[path_last]})" if auto else names[path_last])
-    sched = s.sched_info() if s.path()[0] != 1 else None
+    sched = s.sched_info() if s.path()[0] not in (1, 4) else None
     st = s.stats(allow_nonfinite=True)
     cells = MX * MY
     value = cells * a.steps / wall

@@ -371,7 +371,10 @@ __global__ void __launch_bounds__(BX * BY) k_phase_split(const DevParams<R>* __r
       code[(size_t)x * p.pitch + y] = (signed char)(zero ? onehot_phase<P>(ph) : -1);
     }
   }
-  if (__syncthreads_or(heavy) && threadIdx.x == 0 && threadIdx.y == 0) list[atomicAdd(&cnt[0], 1)] = by * gridDim.x + bx;
+  if (__syncthreads_or(heavy) && threadIdx.x == 0 && threadIdx.y == 0) {
+    const int slot = atomicAdd(&cnt[0], 1);
+    if (slot < (int)(gridDim.x * gridDim.y)) list[slot] = by * gridDim.x + bx;  // a list holds every block at most
+  }
 }
 
 // mu sweep of a split step: the cells (block shape of k_conc_lds / k_conc_march
@@ -399,7 +402,10 @@ __global__ void __launch_bounds__(BX * BY) k_conc_split(const DevParams<R>* __re
     if (cc >= 0) bulk_cell<P, C, R>(p, phi, mu, mulo, phi_o, mu_o, mulo_o, x, y);
     else heavy = 1;
   }
-  if (__syncthreads_or(heavy) && threadIdx.x == 0 && threadIdx.y == 0) list[atomicAdd(&cnt[1], 1)] = by * gridDim.x + bx;
+  if (__syncthreads_or(heavy) && threadIdx.x == 0 && threadIdx.y == 0) {
+    const int slot = atomicAdd(&cnt[1], 1);
+    if (slot < (int)(gridDim.x * gridDim.y)) list[slot] = by * gridDim.x + bx;
+  }
 }
 
 // Sparse two-pass engine for interface tiles.  k_phase_tiles computes

@@ -57,7 +57,7 @@ __device__ __forceinline__ bool blk_pick(const int* __restrict__ list, const int
     blk_xy(bx, by);
     return true;
   }
-  if ((int)blockIdx.x >= *count) return false;
+  if ((int)blockIdx.x >= min(*count, (int)gridDim.x)) return false;  // the 1-D grid spans every block
   const int l = list[blockIdx.x];
   bx = l % gx;
   by = l / gx;
