@@ -364,11 +364,13 @@ __global__ void __launch_bounds__(BX * BY) k_phase_split(const DevParams<R>* __r
       project_simplex<P>(ph, act, dp);
       bool zero = true;
 #pragma unroll
-      for (int a = 0; a < P; a++) {
-        dphi[idx(p, a, x, y)] = dp[a];
-        zero = zero && dp[a] == R(0) && !signbit(dp[a]);
+      for (int a = 0; a < P; a++) zero = zero && dp[a] == R(0) && !signbit(dp[a]);
+      const int cd = zero ? onehot_phase<P>(ph) : -1;
+      code[(size_t)x * p.pitch + y] = (signed char)cd;
+      if (cd < 0) {  // a code >= 0 stands for delta-phi +0 (load_dphi); the planes are left as they are
+#pragma unroll
+        for (int a = 0; a < P; a++) dphi[idx(p, a, x, y)] = dp[a];
       }
-      code[(size_t)x * p.pitch + y] = (signed char)(zero ? onehot_phase<P>(ph) : -1);
     }
   }
   if (__syncthreads_or(heavy) && threadIdx.x == 0 && threadIdx.y == 0) {

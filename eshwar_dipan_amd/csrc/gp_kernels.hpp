@@ -89,6 +89,21 @@ __device__ __forceinline__ void write_with_ghosts(const DevParams<R>& p, int f, 
     for (int j = 0; j < ny; j++) base[(size_t)tx[i] * p.pitch + ty[j]] = v;
 }
 
+// Delta-phi of cell (x, y).  Given the split path's codes, a cell with code
+// >= 0 has delta-phi +0 in every phase (the code is written only then), and
+// the light phase sweep leaves its planes unwritten.
+template <int P, class R>
+__device__ __forceinline__ void load_dphi(const DevParams<R>& p, const R* __restrict__ dphi,
+                                          const signed char* __restrict__ code, int x, int y, R (&d)[P]) {
+  if (code && code[(size_t)x * p.pitch + y] >= 0) {
+#pragma unroll
+    for (int a = 0; a < P; a++) d[a] = R(0);
+  } else {
+#pragma unroll
+    for (int a = 0; a < P; a++) d[a] = dphi[idx(p, a, x, y)];
+  }
+}
+
 template <int P, class R>
 __device__ __forceinline__ void load3x3(const DevParams<R>& p, const R* __restrict__ phi, int x, int y, R (&v)[P][3][3]) {
 #pragma unroll
@@ -273,7 +288,8 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
                                                               R* __restrict__ phi_o, R* __restrict__ mu_o,
                                                               R* __restrict__ mulo_o, R tb = R(0), int tsh = 0,
                                                               const int* __restrict__ list = nullptr,
-                                                              const int* __restrict__ count = nullptr, int gx = 0) {
+                                                              const int* __restrict__ count = nullptr, int gx = 0,
+                                                              const signed char* __restrict__ code = nullptr) {
   const DevParams<R>& p = *prm;
   const int lane = threadIdx.x;
   int bx, by;
@@ -288,8 +304,7 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
   R v[P][3][3];
   load3x3<P>(p, phi, x0, yc, v);
   R dc[P], jb[C];
-#pragma unroll
-  for (int a = 0; a < P; a++) dc[a] = dphi[idx(p, a, x0, yc)];
+  load_dphi<P>(p, dphi, code, x0, yc, dc);
   {
     Nbhd<P, R> n;
     make_nbhd<P>(p, v, n);
@@ -298,8 +313,8 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
     for (int a = 0; a < P; a++) {
       phb[a] = v[a][0][1];
       phc[a] = v[a][1][1];
-      db[a] = dphi[idx(p, a, x0 - 1, yc)];
     }
+    load_dphi<P>(p, dphi, code, x0 - 1, yc, db);
 #pragma unroll
     for (int k = 0; k < C; k++) {
       mb[k] = mu[idx(p, k, x0 - 1, yc)];
@@ -328,9 +343,9 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
       phc[a] = v[a][1][1];
       phf[a] = v[a][2][1];
       phr[a] = v[a][1][2];
-      df[a] = dphi[idx(p, a, x + 1, yc)];
-      dr[a] = dphi[idx(p, a, x, yc + 1)];
     }
+    load_dphi<P>(p, dphi, code, x + 1, yc, df);
+    load_dphi<P>(p, dphi, code, x, yc + 1, dr);
     R mc[C], mf[C], mb[C], mr[C], ml[C];
 #pragma unroll
     for (int k = 0; k < C; k++) {
@@ -366,7 +381,8 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
                                                             R* __restrict__ phi_o, R* __restrict__ mu_o,
                                                             R* __restrict__ mulo_o, R tb = R(0), int tsh = 0,
                                                             const int* __restrict__ list = nullptr,
-                                                            const int* __restrict__ count = nullptr, int gx = 0) {
+                                                            const int* __restrict__ count = nullptr, int gx = 0,
+                                                            const signed char* __restrict__ code = nullptr) {
   __shared__ R jx[C][BY][BX];
   const DevParams<R>& p = *prm;
   const int lane = threadIdx.x, w = threadIdx.y;
@@ -389,10 +405,10 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
     phc[a] = v[a][1][1];
     phf[a] = v[a][2][1];
     phr[a] = v[a][1][2];
-    dc[a] = dphi[idx(p, a, xc, yc)];
-    df[a] = dphi[idx(p, a, xc + 1, yc)];
-    dr[a] = dphi[idx(p, a, xc, yc + 1)];
   }
+  load_dphi<P>(p, dphi, code, xc, yc, dc);
+  load_dphi<P>(p, dphi, code, xc + 1, yc, df);
+  load_dphi<P>(p, dphi, code, xc, yc + 1, dr);
   R mc[C], mf[C], mb[C], mr[C], ml[C];
 #pragma unroll
   for (int k = 0; k < C; k++) {
@@ -413,10 +429,8 @@ __global__ void __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(GP
   if (w == 0) {  // uniform over the wave
     R phb[P], db[P];
 #pragma unroll
-    for (int a = 0; a < P; a++) {
-      phb[a] = v[a][0][1];
-      db[a] = dphi[idx(p, a, xc - 1, yc)];
-    }
+    for (int a = 0; a < P; a++) phb[a] = v[a][0][1];
+    load_dphi<P>(p, dphi, code, xc - 1, yc, db);
     face_jat<P, C, R, NI>(p, n.gXb, n.cYb, n.cYc, phb, phc, mb, mc, db, dc, true, jb, Tc, Tc);
   }
   __syncthreads();

@@ -234,10 +234,12 @@ int run_split(gp_ctx *c, long nsteps) {
                        phi_o, mu_o, mulo_o, lc, cnt);
     if constexpr (Ops<R>::Exact)
       hipLaunchKernelGGL((k_conc_lds<P, C, R>), dim3(gc.x * gc.y), blk, 0, c->stream, prm, phi, mu, (const R *)dphi,
-                         mulo, phi_o, mu_o, mulo_o, R(0), 0, (const int *)lc, (const int *)(cnt + 1), (int)gc.x);
+                         mulo, phi_o, mu_o, mulo_o, R(0), 0, (const int *)lc, (const int *)(cnt + 1), (int)gc.x,
+                         (const signed char *)c->dcode);
     else
       hipLaunchKernelGGL((k_conc_march<P, C, R>), dim3(gc.x * gc.y), blk, 0, c->stream, prm, phi, mu, (const R *)dphi,
-                         mulo, phi_o, mu_o, mulo_o, R(0), 0, (const int *)lc, (const int *)(cnt + 1), (int)gc.x);
+                         mulo, phi_o, mu_o, mulo_o, R(0), 0, (const int *)lc, (const int *)(cnt + 1), (int)gc.x,
+                         (const signed char *)c->dcode);
     c->cur ^= 1;
   }
   c->flags_valid = 0;
