@@ -230,10 +230,10 @@ __global__ void __launch_bounds__(256) GP_HEAVY k_fused(const DevParams<R>* __re
 // and four neighbours' delta-phi are +0: the antitrapping fluxes and the dh/dt
 // source are exact zeros and the five mobilities are equal, so the update is the
 // bulk branch of cell_update with those zeros -- bit-identical to conc_cell.
-template <int P, int C, class R>
+template <int P, int C, class R, bool NI = false>
 __device__ __forceinline__ void bulk_cell(const DevParams<R>& p, const R* __restrict__ phi, const R* __restrict__ mu,
                                           const R* __restrict__ mulo, R* __restrict__ phi_o, R* __restrict__ mu_o,
-                                          R* __restrict__ mulo_o, int x, int y) {
+                                          R* __restrict__ mulo_o, int x, int y, R T = R(0)) {
   R ph[P], zp[P], mc[C], mf[C], mb[C], mr[C], ml[C], z[C];
 #pragma unroll
   for (int a = 0; a < P; a++) {
@@ -262,7 +262,7 @@ __device__ __forceinline__ void bulk_cell(const DevParams<R>& p, const R* __rest
     gml[k] = div_dy(mc[k] - ml[k], p);
   }
   R pho[P], muo[C], dmu[C];
-  cell_update<P, C>(p, ph, mc, zp, z, z, z, z, Dm, Dm, Dm, Dm, gmf, gmb, gmr, gml, pho, muo, dmu);
+  cell_update<P, C, R, NI>(p, ph, mc, zp, z, z, z, z, Dm, Dm, Dm, Dm, gmf, gmb, gmr, gml, pho, muo, dmu, T);
 #pragma unroll
   for (int a = 0; a < P; a++) write_with_ghosts(p, 0, phi_o + (size_t)a * p.plane, x, y, pho[a]);
   if constexpr (Ops<R>::Exact) {
@@ -381,12 +381,14 @@ __global__ void __launch_bounds__(BX * BY) k_phase_split(const DevParams<R>* __r
 
 // mu sweep of a split step: the cells (block shape of k_conc_lds / k_conc_march
 // with one row per wave) whose 3x3 block carries one code a >= 0 take bulk_cell.
-template <int P, int C, class R>
+// With NI, at the row temperature the dense kernel uses (tb, tsh as there).
+template <int P, int C, class R, bool NI = false>
 __global__ void __launch_bounds__(BX * BY) k_conc_split(const DevParams<R>* __restrict__ prm, const R* __restrict__ phi,
                                                         const R* __restrict__ mu, const R* __restrict__ mulo,
                                                         const signed char* __restrict__ code, R* __restrict__ phi_o,
                                                         R* __restrict__ mu_o, R* __restrict__ mulo_o,
-                                                        int* __restrict__ list, int* __restrict__ cnt) {
+                                                        int* __restrict__ list, int* __restrict__ cnt, R tb = R(0),
+                                                        int tsh = 0) {
   const DevParams<R>& p = *prm;
   int bx, by;
   blk_xy(bx, by);
@@ -401,7 +403,7 @@ __global__ void __launch_bounds__(BX * BY) k_conc_split(const DevParams<R>* __re
     for (int u = -1; u <= 1; u++)
 #pragma unroll
       for (int w = -1; w <= 1; w++) cc = code[(size_t)(x + u) * p.pitch + y + w] == cc ? cc : -1;
-    if (cc >= 0) bulk_cell<P, C, R>(p, phi, mu, mulo, phi_o, mu_o, mulo_o, x, y);
+    if (cc >= 0) bulk_cell<P, C, R, NI>(p, phi, mu, mulo, phi_o, mu_o, mulo_o, x, y, NI ? temp_row(p, tb, y, true, tsh) : R(0));
     else heavy = 1;
   }
   if (__syncthreads_or(heavy) && threadIdx.x == 0 && threadIdx.y == 0) {

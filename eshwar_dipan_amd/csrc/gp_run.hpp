@@ -210,12 +210,12 @@ int run_sparse(gp_ctx *c, long nsteps) {
 // 109-111)
 inline double temp_bottom(const gp_params &h, long shift_off, long t) { return gp_temp_bottom(&h, shift_off, t); }
 
-// Split two-pass steps (path 4, isothermal): per sweep a light kernel over the
-// grid that finishes the cells with an exact shortcut and lists the blocks
-// that hold others, then the dense kernel over the listed blocks only.  The
-// dense kernels take a 1-D grid of every block; blocks past the list's count
-// return at once.
-template <int P, int C, class R>
+// Split two-pass steps (path 4): per sweep a light kernel over the grid that
+// finishes the cells with an exact shortcut and lists the blocks that hold
+// others, then the dense kernel over the listed blocks only.  The dense
+// kernels take a 1-D grid of every block; blocks past the list's count return
+// at once.  Non-isothermal steps take the temperature arguments of run_steps.
+template <int P, int C, class R, bool NI>
 int run_split(gp_ctx *c, long nsteps) {
   static_assert(KM == 1, "the split mu sweep assumes one row per wave");
   const DevParams<R> *prm = (const DevParams<R> *)c->dprm;
@@ -223,25 +223,34 @@ int run_split(gp_ctx *c, long nsteps) {
   const dim3 gp = phase_grid(c->rx, c->ry);
   const dim3 gc(((int)c->hp.mesh_y + MW - 1) / MW, ((int)c->hp.mesh_x + BY - 1) / BY);
   int *cnt = c->dsplit, *lp = cnt + 2, *lc = lp + gp.x * gp.y;
+  const signed char *code = (const signed char *)c->dcode;
   for (long t = 0; t < nsteps; t++) {
     const R *phi = (const R *)c->phi[c->cur], *mu = (const R *)c->mu[c->cur], *mulo = (const R *)c->mulo[c->cur];
     R *phi_o = (R *)c->phi[c->cur ^ 1], *mu_o = (R *)c->mu[c->cur ^ 1], *mulo_o = (R *)c->mulo[c->cur ^ 1];
     R *dphi = (R *)c->dphi;
+    R tb = R(0);
+    int tmode = 0, tsh = 0;
+    if constexpr (NI) {  // as in run_steps
+      const long k = c->tstep + t;
+      tsh = t == 0 ? (int)c->shift_pend : 0;
+      tb = (R)temp_bottom(c->hp, c->shift_off - tsh, k);
+      tmode = (k == 0 ? 1 : 0) | (tsh << 1);
+    }
     hipLaunchKernelGGL((k_phase_split<P, R>), gp, blk, 0, c->stream, prm, phi, dphi, c->dcode, lp, cnt);
-    hipLaunchKernelGGL((k_phase<P, C, R>), dim3(gp.x * gp.y), blk, 0, c->stream, prm, phi, mu, dphi, R(0), 0,
+    hipLaunchKernelGGL((k_phase<P, C, R, NI>), dim3(gp.x * gp.y), blk, 0, c->stream, prm, phi, mu, dphi, tb, tmode,
                        (const int *)lp, (const int *)cnt, (int)gp.x);
-    hipLaunchKernelGGL((k_conc_split<P, C, R>), gc, blk, 0, c->stream, prm, phi, mu, mulo, (const signed char *)c->dcode,
-                       phi_o, mu_o, mulo_o, lc, cnt);
+    hipLaunchKernelGGL((k_conc_split<P, C, R, NI>), gc, blk, 0, c->stream, prm, phi, mu, mulo, code, phi_o, mu_o,
+                       mulo_o, lc, cnt, tb, tsh);
     if constexpr (Ops<R>::Exact)
-      hipLaunchKernelGGL((k_conc_lds<P, C, R>), dim3(gc.x * gc.y), blk, 0, c->stream, prm, phi, mu, (const R *)dphi,
-                         mulo, phi_o, mu_o, mulo_o, R(0), 0, (const int *)lc, (const int *)(cnt + 1), (int)gc.x,
-                         (const signed char *)c->dcode);
+      hipLaunchKernelGGL((k_conc_lds<P, C, R, NI>), dim3(gc.x * gc.y), blk, 0, c->stream, prm, phi, mu, (const R *)dphi,
+                         mulo, phi_o, mu_o, mulo_o, tb, tsh, (const int *)lc, (const int *)(cnt + 1), (int)gc.x, code);
     else
-      hipLaunchKernelGGL((k_conc_march<P, C, R>), dim3(gc.x * gc.y), blk, 0, c->stream, prm, phi, mu, (const R *)dphi,
-                         mulo, phi_o, mu_o, mulo_o, R(0), 0, (const int *)lc, (const int *)(cnt + 1), (int)gc.x,
-                         (const signed char *)c->dcode);
+      hipLaunchKernelGGL((k_conc_march<P, C, R, NI>), dim3(gc.x * gc.y), blk, 0, c->stream, prm, phi, mu,
+                         (const R *)dphi, mulo, phi_o, mu_o, mulo_o, tb, tsh, (const int *)lc, (const int *)(cnt + 1),
+                         (int)gc.x, code);
     c->cur ^= 1;
   }
+  if (nsteps > 0) c->shift_pend = 0;
   c->flags_valid = 0;
   GP_HIP_CHECK(hipGetLastError());
   return GP_OK;
@@ -249,7 +258,7 @@ int run_split(gp_ctx *c, long nsteps) {
 
 template <int P, int C, class R>
 int run_steps(gp_ctx *c, long nsteps) {
-  if (c->path == 4) return run_split<P, C, R>(c, nsteps);
+  if (c->path == 4) return c->hp.isothermal ? run_split<P, C, R, false>(c, nsteps) : run_split<P, C, R, true>(c, nsteps);
   if (c->path == 0 || c->path == 2 || c->path == 3) {
     // the fused kernel (paths 0, 2) and ensembles of grids use their own constraints
     int s = (c->path == 3 && shape_allowed<P>(c->shape)) ? c->shape : 0;

@@ -288,8 +288,9 @@ static int ctx_create(const gp_params *p, int dtype, bool own_stream, gp_ctx **o
   }
   c->stream = c->own_stream;
   c->cur = 0;
-  c->path = p->isothermal ? 0 : 1;        // non-isothermal runs step on the two-pass path only
-  c->use_graphs = p->isothermal ? 1 : 0;  // and take per-step arguments (no graphs)
+  c->path = 1;                            // automatic choice from the two-pass path on
+  c->auto_mode = 1;
+  c->use_graphs = p->isothermal ? 1 : 0;  // non-isothermal steps take per-step arguments (no graphs)
   c->graph_path = -1;
   c->shape = 0;
   c->shape_set = -1;
@@ -321,12 +322,8 @@ int gp_set_graphs(gp_ctx *c, int on) {
 
 int gp_set_path(gp_ctx *c, int path) {
   if (!c || path < -1 || path > 4 || (c->nmember && path != 3 && path != -1)) return GP_E_ARG;
-  if (!c->hp.isothermal) {  // non-isothermal runs step on the two-pass path only
-    if (path != 1 && path != -1) return GP_E_UNSUPPORTED;
-    c->auto_mode = 0;
-    c->path = 1;
-    return GP_OK;
-  }
+  if (!c->hp.isothermal && path != 1 && path != 4 && path != -1)
+    return GP_E_UNSUPPORTED;  // non-isothermal runs step on the two-pass or split two-pass path
   c->auto_mode = path < 0;
   c->auto_count = 0;
   c->auto_next = 0;
@@ -478,6 +475,44 @@ static int retune(gp_ctx *c, long *left) {
   return rc;
 }
 
+// Automatic choice without graphs (graphs off, the default stream, or a
+// non-isothermal context): one timed 32-step block of eager launches of each
+// candidate, real steps of the run.  Counts the steps it takes in tstep.
+static int retune_eager(gp_ctx *c, long *left) {
+  const int iso[4] = {1, 4, 0, 3}, ni[2] = {1, 4};
+  const int *cand = c->hp.isothermal ? iso : ni;
+  const int ncand = c->nmember ? 1 : (c->hp.isothermal ? 4 : 2);
+  double best = 1e300;
+  int pick = c->path;
+  hipEvent_t t0, t1;
+  if (hipEventCreate(&t0) != hipSuccess) return GP_E_HIP;
+  if (hipEventCreate(&t1) != hipSuccess) {
+    (void)hipEventDestroy(t0);
+    return GP_E_HIP;
+  }
+  int rc = GP_OK;
+  for (int i = 0; i < ncand && *left >= GP_GRAPH_STEPS; i++) {
+    c->path = c->nmember ? 3 : cand[i];
+    c->shape = 0;
+    if (hipEventRecord(t0, c->stream) != hipSuccess) { rc = GP_E_HIP; break; }
+    if ((rc = dispatch_steps(c, GP_GRAPH_STEPS))) break;
+    if (hipEventRecord(t1, c->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess) { rc = GP_E_HIP; break; }
+    c->tstep += GP_GRAPH_STEPS;
+    c->shift_pend = 0;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    if (ms < best) {
+      best = ms;
+      pick = c->path;
+    }
+    *left -= GP_GRAPH_STEPS;
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  c->path = pick;
+  return rc;
+}
+
 int gp_step(gp_ctx *c, long nsteps) { return gp_step_impl(c, nsteps, true); }
 
 }  // extern "C"
@@ -506,10 +541,35 @@ int gp_step_impl(gp_ctx *c, long nsteps, bool timed) {
       left -= GP_GRAPH_STEPS;
     }
   }
+  long counted = 0;  // steps already added to tstep (the eager automatic choice)
+  if (!rc && c->auto_mode && !(c->use_graphs && c->stream)) {
+    while (!rc && left >= GP_GRAPH_STEPS) {
+      if (c->auto_count >= c->auto_next && left >= 3 * GP_GRAPH_STEPS) {
+        const int before = c->path;
+        const bool first = c->auto_count == 0;
+        const long l0 = left;
+        rc = retune_eager(c, &left);
+        counted += l0 - left;
+        c->auto_count++;
+        c->auto_gap = (!first && c->path == before) ? std::min(2 * c->auto_gap, 8 * c->auto_period) : c->auto_period;
+        c->auto_next = c->auto_count + c->auto_gap;
+        continue;
+      }
+      // up to the next re-timing in one dispatch
+      const long blocks = std::max(1L, std::min(left / GP_GRAPH_STEPS, c->auto_next - c->auto_count));
+      const long n = blocks * GP_GRAPH_STEPS;
+      rc = dispatch_steps(c, n);
+      c->tstep += n;
+      c->shift_pend = 0;
+      counted += n;
+      left -= n;
+      c->auto_count += blocks;
+    }
+  }
   if (!rc && left > 0) rc = dispatch_steps(c, left);
   if (rc) return rc;
   if (nsteps > 0) c->stepped = 1;
-  c->tstep += nsteps;
+  c->tstep += nsteps - counted;
   if (nsteps > 0) c->shift_pend = 0;
   for (int i = 0; i < c->nmember; i++) {
     c->member[i]->cur = c->cur;

@@ -173,16 +173,17 @@ int  gp_set_stream(gp_ctx *ctx, void *hip_stream);
 int  gp_last_step_ms(gp_ctx *ctx, double *ms);
 
 /* Kernel path of a context: 0 = fused LDS kernel on interface tiles plus a
-   diffusion-only kernel on bulk tiles (default), 1 = two-pass kernels (phase
+   diffusion-only kernel on bulk tiles, 1 = two-pass kernels (phase
    kernel + concentration kernel over the whole grid), 2 = fused LDS kernel on
    every tile, 3 = two-pass kernels on interface tiles (cells sorted in LDS into
    interface and bulk cells) plus the bulk kernel, 4 = split two-pass: per sweep
    a light kernel finishes the bulk cells and lists the blocks holding others,
-   which the two-pass kernels then update, -1 = automatic: at the start, after
-   64 blocks of 32 steps and then at doubling gaps, one block of each of paths
-   1, 4, 0 and 3 is timed and the fastest runs until the next re-timing.  All
-   paths give bit-identical binary64 results.  Non-isothermal contexts step on
-   path 1 only (other values: GP_E_UNSUPPORTED). */
+   which the two-pass kernels then update, -1 = automatic (the default): at the
+   start, after 64 blocks of 32 steps and then at doubling gaps, one block of
+   each of paths 1, 4, 0 and 3 is timed (as graph replays, or as direct
+   launches where graphs are off) and the fastest runs until the next
+   re-timing.  All paths give bit-identical binary64 results.  Non-isothermal
+   contexts step on paths 1 and 4 only (other values: GP_E_UNSUPPORTED). */
 int  gp_set_path(gp_ctx *ctx, int path);
 /* The path in use (after automatic choices) and whether the choice is automatic. */
 int  gp_get_path(gp_ctx *ctx, int *path, int *automatic);

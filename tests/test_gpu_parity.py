@@ -22,8 +22,8 @@ def _cmp(a, b):
 def test_gpu_f64_reproduces_reference(name, path):
     print(f"\n[gpu] golden {name} path={path}", flush=True)
     z, inp, _ = gputil.load_golden(name)
-    if not inp.p.isothermal and path != 1:
-        # non-isothermal runs step on the two-pass path only; the others refuse
+    if not inp.p.isothermal and path not in (1, 4):
+        # non-isothermal runs step on the two-pass paths only; the others refuse
         with pytest.raises(gpstep.GPError):
             gpstep.Solver(inp, "f64", path=path)
         return
@@ -79,8 +79,10 @@ def test_gpu_f64_matches_oracle(name, spec, init, steps):
     inp, phi0, mu0 = _oracle_case(spec, init, seed=sum(map(ord, name)))
     ophi, omu = gputil.oracle_run(inp.p, phi0, mu0, steps, nthreads=0)
     ni = not inp.p.isothermal
-    for path in ((1,) if ni else (1, 0, 2, 3, 4)):
-        s = gpstep.Solver(inp, "f64", path=path)
+    # -1: the automatic choice; where graphs are off (always for a
+    # non-isothermal run) its timed blocks are direct launches
+    for path in ((1, 4, -1) if ni else (1, 0, 2, 3, 4, -1)):
+        s = gpstep.Solver(inp, "f64", path=path, graphs=path != -1)
         s.set_state(phi0, mu0)
         if ni:  # the temperature follows the step count across calls
             s.step(steps // 3)
@@ -303,7 +305,7 @@ def _frame_case(kind):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind,path", [("iso_p3c2", 1), ("iso_p3c2", 0), ("iso_p3c2", 3), ("iso_p3c2", 4), ("ni_p3c2", 1),
-                                       ("ni_p2c1", 1)])
+                                       ("ni_p3c2", 4), ("ni_p2c1", 1)])
 def test_gpu_moving_frame_matches_oracle(kind, path):
     """Moving frame (Shift): check_SHIFT and apply_shiftY every 100 steps over
     400 steps of a 96x128 ternary front; binary64 states equal the oracle's
@@ -357,7 +359,7 @@ def test_gpu_split_path_f32_and_refusals():
     """The split two-pass path (4) in binary32 stays within binary32 rounding of
     the two-pass path (1): its bulk shortcut evaluates the same bulk update
     through another expression tree, which -ffp-contract=fast may contract
-    differently.  Non-isothermal runs refuse the path."""
+    differently."""
     print("\n[gpu] split path f32 vs two-pass f32, 256x256 P=4 C=2, 200 steps", flush=True)
     inp, phi0, mu0 = _oracle_case(dict(P=4, C=2, mesh=(256, 256), aniso=0.02, bc=(3, 3, 3, 3), gabc=1.0),
                                   dict(nseeds=32, r=(6, 12), diffuse=0.0, noise=1e-3), seed=7)
@@ -371,6 +373,3 @@ def test_gpu_split_path_f32_and_refusals():
     emu = float(np.abs(out[4][1] - out[1][1]).max() / max(np.ptp(out[1][1]), 1e-30))
     print(f"  max|dphi|={ephi:.3e} rel max|dmu|={emu:.3e}", flush=True)
     assert ephi < 1e-5 and emu < 1e-5
-    ni = gputil.make_params(P=2, C=1, mesh=(64, 64), tempgrady="{0.95, 0.06, 100.0, 4, 2.0}")
-    with pytest.raises(gpstep.GPError):
-        gpstep.Solver(ni, "f64", path=4)
