@@ -325,6 +325,30 @@ __device__ __forceinline__ bool phase_active(const DevParams<R>& p, const R* __r
   return any;
 }
 
+// Uniform 5-point neighbourhood in every phase: then every gradient of the
+// phase sweep is an exact zero, so no phase is active.  A conservative form of
+// phase_active without arithmetic (all loads first, then comparisons): a cell
+// that fails it goes to the dense kernel, which decides activity exactly.
+template <int P, class R>
+__device__ __forceinline__ bool phase_uniform(const DevParams<R>& p, const R* __restrict__ phi, int x, int y, R* ph) {
+  R c[P], f[P], b[P], r[P], l[P];
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    c[a] = phi[idx(p, a, x, y)];
+    f[a] = phi[idx(p, a, x + 1, y)];
+    b[a] = phi[idx(p, a, x - 1, y)];
+    r[a] = phi[idx(p, a, x, y + 1)];
+    l[a] = phi[idx(p, a, x, y - 1)];
+  }
+  bool u = true;
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    u = u && f[a] == c[a] && b[a] == c[a] && r[a] == c[a] && l[a] == c[a];
+    ph[a] = c[a];
+  }
+  return u;
+}
+
 // Split two-pass step (path 4).  The dense two-pass kernels need about 240
 // registers per lane for interface cells, which leaves two waves per SIMD also
 // for bulk cells, whose few loads then go unhidden.  Each sweep of a split step
@@ -350,7 +374,7 @@ __global__ void __launch_bounds__(BX * BY) k_phase_split(const DevParams<R>* __r
   int heavy = 0;
   if (x <= p.rx - 3 && y <= p.ry - 3) {
     R ph[P];
-    if (phase_active<P>(p, phi, x, y, ph)) {
+    if (!phase_uniform<P>(p, phi, x, y, ph)) {  // possibly active: the dense kernel decides
       heavy = 1;
       code[(size_t)x * p.pitch + y] = -1;
     } else {
