@@ -30,6 +30,7 @@ struct gp_ctx {
   long shift_off;       // rows the moving frame has moved since the state was loaded
   long shift_pend;      // rows of a shift not yet followed by a step (temperature of that step)
   int *drow;            // interface position per interior row (allocated on first use)
+  int pow2;             // dx and dy are powers of two (binary64 steps from namespace gpk2)
   int path;             // 0 sparse fused, 1 two-pass, 2 dense fused, 3 sparse two-pass, 4 split two-pass
   hipStream_t stream;
   hipEvent_t ev0, ev1;
@@ -85,6 +86,17 @@ template <int P, int C, class R>
 int run_smooth(gp_ctx *c, long nsweeps);
 
 }  // namespace gpk
+
+// the binary64 steps for spacings that are powers of two (gp_inst_p*.hip
+// compiled with -DGP_POW2 -DGPK_NS=gpk2)
+namespace gpk2 {
+template <int P, int C, class R>
+int run_steps(gp_ctx *c, long nsteps);
+template <int P, int C, class R>
+int run_stats(gp_ctx *c, gp_stats *st);
+template <int P, int C, class R>
+int run_smooth(gp_ctx *c, long nsweeps);
+}  // namespace gpk2
 
 // gp_step without the closing synchronisation when timed is false
 int gp_step_impl(gp_ctx *c, long nsteps, bool timed);

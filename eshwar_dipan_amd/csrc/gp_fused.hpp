@@ -18,7 +18,7 @@
 
 #include "gp_kernels.hpp"
 
-namespace gpk {
+namespace GPK_NS {
 
 template <int P, int C, class R, int TX, int TY>
 struct FusedSmem {
@@ -611,4 +611,4 @@ __global__ void __launch_bounds__(256) k_flags(const Inst<R>* __restrict__ inst,
   if (threadIdx.x == 0) flag[tile] = f == -2 ? -1 : f;
 }
 
-}  // namespace gpk
+}  // namespace GPK_NS

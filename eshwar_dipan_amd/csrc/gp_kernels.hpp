@@ -28,7 +28,7 @@
 #define GP_CONC_WAVES GP_WAVES_PER_EU
 #endif
 
-namespace gpk {
+namespace GPK_NS {
 
 constexpr int BX = 64;  // threads along y (contiguous)
 constexpr int BY = 4;   // threads along x
@@ -723,4 +723,4 @@ __global__ void __launch_bounds__(BX * BY) GP_HEAVY k_smooth(const DevParams<R>*
   }
 }
 
-}  // namespace gpk
+}  // namespace GPK_NS

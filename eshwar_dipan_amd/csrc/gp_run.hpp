@@ -8,7 +8,7 @@
 #include "gp_ctx.hpp"
 #include "gp_fused.hpp"
 
-namespace gpk {
+namespace GPK_NS {
 
 // Tile of the fused kernel: the largest of (8,64), (4,64), (8,32), (4,32),
 // (2,32) interior cells whose LDS staging fits in 60 KiB.
@@ -351,9 +351,9 @@ int run_stats(gp_ctx *c, gp_stats *st) {
   return GP_OK;
 }
 
-}  // namespace gpk
+}  // namespace GPK_NS
 
-#define GP_INSTANTIATE(P, C, R)                             \
-  template int gpk::run_steps<P, C, R>(gp_ctx *, long);    \
-  template int gpk::run_smooth<P, C, R>(gp_ctx *, long);   \
-  template int gpk::run_stats<P, C, R>(gp_ctx *, gp_stats *);
+#define GP_INSTANTIATE(P, C, R)                                \
+  template int GPK_NS::run_steps<P, C, R>(gp_ctx *, long);    \
+  template int GPK_NS::run_smooth<P, C, R>(gp_ctx *, long);   \
+  template int GPK_NS::run_stats<P, C, R>(gp_ctx *, gp_stats *);

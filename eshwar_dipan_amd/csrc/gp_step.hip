@@ -264,6 +264,7 @@ static int ctx_create(const gp_params *p, int dtype, bool own_stream, gp_ctx **o
     ok = ok && h;
     if (ok) {
       fill_devparams(*p, c->rx, c->ry, c->pitch, c->plane, *h);
+      c->pow2 = h->pow2_dx && h->pow2_dy;
       ok = hipMalloc(&c->dprm, sizeof(*h)) == hipSuccess && hipMemcpy(c->dprm, h, sizeof(*h), hipMemcpyHostToDevice) == hipSuccess;
     }
     delete h;
@@ -377,7 +378,9 @@ static int dispatch_steps(gp_ctx *c, long nsteps) {
   int rc = GP_E_UNSUPPORTED;
 #define GP_DISPATCH_STEP(P_, C_)                                                                   \
   if (c->P == P_ && c->C == C_)                                                                    \
-    rc = c->dtype == GP_F64 ? gpk::run_steps<P_, C_, double>(c, nsteps) : gpk::run_steps<P_, C_, float>(c, nsteps);
+    rc = c->dtype == GP_F32 ? gpk::run_steps<P_, C_, float>(c, nsteps)                              \
+         : c->pow2          ? gpk2::run_steps<P_, C_, double>(c, nsteps)                             \
+                            : gpk::run_steps<P_, C_, double>(c, nsteps);
   GP_PC_LIST(GP_DISPATCH_STEP)
 #undef GP_DISPATCH_STEP
   return rc;

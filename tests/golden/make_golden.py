@@ -131,6 +131,10 @@ CASES = [
     dict(name="p3c2_tempgrady_ternary_periodic", P=3, C=2, mesh=(40, 48), solver="TERNARY", bc=(1, 1, 3, 3),
          tempgrady="{0.94, 0.1, 60.0, 2, 2.0}", init="fill",
          fill=["FILLCYLINDER = {0, 12, 14, 0, 0, 6};", "FILLCYLINDER = {1, 28, 30, 0, 0, 7};"], steps=60, save=[1, 60]),
+    # spacing that is not a power of two (divisions by dx, dy stay divisions)
+    dict(name="p3c2_dx08_ternary_fill", P=3, C=2, mesh=(40, 48), solver="TERNARY", aniso=0.02,
+         keys={"DELTA_X": "0.8", "DELTA_Y": "0.8"}, init="fill",
+         fill=["FILLCYLINDER = {0, 12, 14, 0, 0, 6};", "FILLCYLINDER = {1, 27, 31, 0, 0, 7};"], steps=60, save=[1, 60]),
     dict(name="p4c3_general_cube", P=4, C=3, mesh=(32, 32), solver="GENERAL", A_off=0.2,
          init="fill", fill=["FILLCUBE = {0, 4, 4, 0, 12, 12, 0};", "FILLCYLINDER = {1, 22, 20, 0, 0, 6};",
                             "FILLSPHERE = {2, 10, 24, 0, 5};"], steps=30, save=[1, 30]),
@@ -144,6 +148,7 @@ def build_case(c):
     keys = synth.base_params(P, C, (MX, MY))
     keys["NTIMESTEPS"] = str(c["steps"])
     keys["SAVET"] = "1"
+    keys.update(c.get("keys", {}))
     if c.get("nsmooth"):
         keys["NSMOOTH"] = str(c["nsmooth"])
     if c.get("aniso"):

@@ -52,6 +52,12 @@ ORACLE_CASES = [
      dict(nseeds=1, r=(6, 6), diffuse=1.0), 300),
     ("c2_1024_p4c2_full_size", dict(P=4, C=2, mesh=(1024, 1024), aniso=0.02, bc=(3, 3, 3, 3), gabc=1.0),
      dict(nseeds=32, r=(6, 12), diffuse=0.0, noise=1e-3), 20),
+    # spacings: not a power of two (run-time division) and a power of two other than 1
+    ("c2_128_p4c2_dx08", dict(P=4, C=2, mesh=(128, 128), aniso=0.02, bc=(3, 3, 3, 3), gabc=1.0,
+                              extra=("DELTA_X = 0.8;", "DELTA_Y = 0.8;")),
+     dict(nseeds=12, r=(5, 9), diffuse=0.0, noise=1e-3), 150),
+    ("c1_96_dx05", dict(P=2, C=1, mesh=(96, 96), aniso=0.02, extra=("DELTA_X = 0.5;", "DELTA_Y = 0.5;")),
+     dict(nseeds=3, r=(5, 8), diffuse=1.0), 150),
     # non-isothermal: temperature gradient in y moving with the Tempgrady velocity
     ("ni_192_p3c2_tempgrady_periodic_y", dict(P=3, C=2, mesh=(160, 192), aniso=0.02, bc=(1, 1, 3, 3),
                                               tempgrady="{0.95, 0.08, 120.0, 10, 2.5}"),
