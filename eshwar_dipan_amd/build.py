@@ -49,9 +49,11 @@ def _units():
                   [HIPCC, *HIP_FLAGS, "-ffp-contract=off", "-DGP_REAL=double", "-c", src]))
         u.append((os.path.join(BUILD, f"gp_inst_p{P}_f32.o"),
                   [HIPCC, *HIP_FLAGS, "-ffp-contract=fast", "-DGP_REAL=float", "-c", src]))
-        # binary64 steps for spacings that are powers of two (namespace gpk2)
-        u.append((os.path.join(BUILD, f"gp_inst_p{P}_f64p2.o"),
-                  [HIPCC, *HIP_FLAGS, "-ffp-contract=off", "-DGP_REAL=double", "-DGP_POW2", "-DGPK_NS=gpk2", "-c", src]))
+        # binary64 steps for spacings that are powers of two (namespace gpk2), P <= 4
+        if P <= 4:
+            u.append((os.path.join(BUILD, f"gp_inst_p{P}_f64p2.o"),
+                      [HIPCC, *HIP_FLAGS, "-ffp-contract=off", "-DGP_REAL=double", "-DGP_POW2", "-DGPK_NS=gpk2",
+                       "-c", src]))
     return u
 
 
