@@ -248,7 +248,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default: the workload's steps to solution, capped at 2e10 cell updates)")
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default=None, help="c1, c2 (default), c3, c4 (ensemble), c5")
+    ap.add_argument("--workload", default=None,
+                    help="c1, c2 (default), c2ni (c2 in a moving temperature gradient), c3, c4 (ensemble), c5")
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--path", type=int, default=-1,
                     help="-1 automatic (default), 0 sparse fused, 1 two-pass, 2 dense fused, 3 sparse two-pass, "
@@ -320,7 +321,9 @@ def main():
              "sparse: two-pass kernels on interface tiles + bulk kernel",
              "split two-pass: light kernels finish bulk cells, two-pass kernels on the listed interface blocks"]
     path_last, auto = s.path()
-    path_desc = (f"automatic (one timed 32-step block of each of two-pass, split two-pass, sparse fused, sparse two-pass at the start, "
+    cands = ("two-pass, split two-pass, sparse fused, sparse two-pass" if p.isothermal
+             else "two-pass and split two-pass (direct launches: non-isothermal steps are not captured)")
+    path_desc = (f"automatic (one timed 32-step block of each of {cands} at the start, "
                  f"after 64 blocks, then at gaps doubling up to 512 blocks while the choice holds; "
                  f"last: {names[path_last]})" if auto else names[path_last])
     sched = s.sched_info() if s.path()[0] not in (1, 4) else None

@@ -15,6 +15,7 @@ follow the benchmark configurations of this path:
   c1  2 phases, binary, 256^2, isotropic, one seed r=10, Neumann, 1000 steps
   c2  3 solids + liquid, ternary, 1024^2, 4-fold anisotropy 0.02, 32 seeds r in
       [6,12] with random labels, mu + N(0, 1e-3), periodic, 10000 steps
+  c2ni  c2 in a temperature gradient along y moving at 2.5 (Tempgrady), 10000 steps
   c3  8 solids + liquid, binary, 4096^2, Voronoi grains from 200 sites, 20000 steps
   c4  ensemble of 2-phase binary instances of uneven sizes (batched path)
   c5  2-phase binary dendrite, 512^2, 4-fold anisotropy 0.02, 2000 steps
@@ -115,6 +116,16 @@ def c2(mesh=1024):
                                  "32 seeds r 6-12, mu + N(0,1e-3), periodic")
 
 
+def c2ni(mesh=1024):
+    """The c2 microstructure in a moving temperature gradient along y
+    (Tempgrady; Neumann along x, periodic along y as in c2's y faces)."""
+    inp = params(4, 2, (mesh, mesh), aniso=0.02, bc=(1, 1, 3, 3), gabc=1.0, T=0.9, gamma=0.5,
+                 tempgrady="{0.9, 0.08, 1000.0, 10, 2.5}")
+    phi, mu = seeded_state(inp, 32, 6, 12, seed=2, noise=1e-3)
+    return inp, phi, mu, 10000, (f"3 solids + liquid, ternary, {mesh}x{mesh}, 4-fold anisotropy 0.02, 32 seeds r 6-12, "
+                                 "mu + N(0,1e-3), temperature 0.9 + 0.08 per 1000 rows moving at 2.5")
+
+
 def c3(mesh=4096):
     inp = params(9, 1, (mesh, mesh), gabc=0.5, T=0.9, gamma=0.5)
     phi, mu = voronoi_state(inp, 200, seed=3, liquid_fraction=0.3)
@@ -156,4 +167,4 @@ def c4(n=512, steps=2000, seed=4):
     return members, steps, desc
 
 
-WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c5": c5}
+WORKLOADS = {"c1": c1, "c2": c2, "c2ni": c2ni, "c3": c3, "c5": c5}
