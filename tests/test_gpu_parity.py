@@ -174,7 +174,7 @@ def _temp_bottom(p, t):
 @pytest.mark.gpu
 def test_gpu_nonisothermal_f32_stats_and_limits():
     """Non-isothermal run (Tempgrady): binary32 close to binary64; the solute sum
-    uses each row's temperature; paths other than two-pass, smoothing, ensembles
+    uses each row's temperature; paths other than the two-pass ones (1, 4), smoothing, ensembles
     and the adjoint refuse such a run."""
     print("\n[gpu] non-isothermal f32 vs f64, stats, refusals, 128x112 dendrite, 200 steps", flush=True)
     inp = gputil.make_params(P=2, C=1, mesh=(128, 112), aniso=0.02, tempgrady="{0.95, 0.06, 100.0, 4, 2.0}")
