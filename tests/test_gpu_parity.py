@@ -85,10 +85,12 @@ def test_gpu_f64_matches_oracle(name, spec, init, steps):
     inp, phi0, mu0 = _oracle_case(spec, init, seed=sum(map(ord, name)))
     ophi, omu = gputil.oracle_run(inp.p, phi0, mu0, steps, nthreads=0)
     ni = not inp.p.isothermal
-    # -1: the automatic choice; where graphs are off (always for a
-    # non-isothermal run) its timed blocks are direct launches
-    for path in ((1, 4, -1) if ni else (1, 0, 2, 3, 4, -1)):
-        s = gpstep.Solver(inp, "f64", path=path, graphs=path != -1)
+    # -1: the automatic choice, with graph replays and, where graphs are off
+    # (always for a non-isothermal run), with timed blocks of direct launches
+    runs = ((1, True), (4, True), (-1, False)) if ni else \
+        ((1, True), (0, True), (2, True), (3, True), (4, True), (-1, True), (-1, False))
+    for path, graphs in runs:
+        s = gpstep.Solver(inp, "f64", path=path, graphs=graphs)
         s.set_state(phi0, mu0)
         if ni:  # the temperature follows the step count across calls
             s.step(steps // 3)
@@ -99,7 +101,8 @@ def test_gpu_f64_matches_oracle(name, spec, init, steps):
         ep, bp = _cmp(phi, ophi)
         em, bm = _cmp(mu, omu)
         moved = float(np.abs(ophi - phi0).max())
-        print(f"  path={path} max|dphi|={ep:.3e} max|dmu|={em:.3e} bitexact={bp and bm} (field moved {moved:.3e})", flush=True)
+        print(f"  path={path} graphs={graphs} max|dphi|={ep:.3e} max|dmu|={em:.3e} bitexact={bp and bm} "
+              f"(field moved {moved:.3e})", flush=True)
         assert moved > 0
         assert ep <= TOL64 and em <= TOL64, (name, path, ep, em)
 
