@@ -334,13 +334,16 @@ def main():
     B = 2 * (P + C) * esize
     kern_s = dev_ms * 1e-3 / a.steps
     achieved = B * cells / kern_s / 1e9
-    traffic = None
+    traffic = reads = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         try:
-            traffic = json.load(open(tf)).get(f"{a.workload}_{a.dtype}_path{a.path}")
+            t = json.load(open(tf))
+            traffic = t.get(f"{a.workload}_{a.dtype}_path{a.path}")
+            # HBM reads alone, where only the read counter run is complete
+            reads = t.get(f"{a.workload}_{a.dtype}_path{path_last}_reads") if traffic is None else None
         except Exception:
-            traffic = None
+            traffic = reads = None
     out = {
         "metric": "cell updates/s + % HBM roofline, FP32 vs FP64 at stated err; adjoint/fwd time",
         "value": value,
@@ -360,6 +363,8 @@ def main():
                    "steps_to_solution": nsteps_full},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     **({"traffic_reads": reads, "traffic_reads_note": "FETCH_SIZE only, scaled, mean over the first "
+                         "2000 steps of the path the run ended on; the WRITE_SIZE run is missing"} if reads else {}),
                      "bytes_per_cell_update": B, "kernel_ms_per_step": kern_s * 1e3},
         "nonfinite": st["nonfinite"],
     }
